@@ -1,0 +1,97 @@
+#!/usr/bin/env python3
+"""Build reference binaries from the reference's own C sources into oracle/_ref/.
+
+Test infrastructure only.  Each binary is one gcc invocation over the
+reference's few source files (src/main.c, src/tools.c, src/methods.c,
+src/threads.c, one problem and one implementation), with the reference's own
+flags (Makefile:46: -O3 -g -D_GNU_SOURCE -D_REENTRANT, -lpthread -lm) and the
+compile-time configuration as -D defines (include/config.h).  Nothing is
+copied out of the reference tree; outputs go to oracle/_ref/ only.
+
+Usage: python oracle/build_ref.py [--ref /root/reference]
+Does nothing (exit 0) when the reference tree is absent.
+"""
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+
+# name -> (kind, problem, impl, defines)
+CONFIGS = {
+    # known answers (footprints + step counts)
+    "seq_bruss_D_n128_dopri54_tol1e-6": ("seq", "bruss2d-mix", "D",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "seq_bruss_PipeDls_n128_dopri54_tol1e-6": ("seq", "bruss2d-mix", "PipeDls",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "seq_bruss_D_n128_verner65_tol1e-6": ("seq", "bruss2d-mix", "D",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="VERNER65")),
+    "seq_bruss_D_n128_defaults": ("seq", "bruss2d-mix", "D", dict(BRUSS_GRID_SIZE=128)),
+    "seq_lv_D_10000_dopri54": ("seq", "lv", "D",
+        dict(LV_COUNT=10000, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "seq_lv_D_10000_verner65_t20": ("seq", "lv", "D",
+        dict(LV_COUNT=10000, T_END="20.0", TOL="1E-8", STEP_LIMIT=0, METHOD="VERNER65")),
+    # CPU baseline: pthreads PipeD4 on the 2000x2000 grid, 200-step cap
+    "pthreads_bruss_PipeD4_n2000_dopri54_sl200": ("pthreads", "bruss2d-mix", "PipeD4",
+        dict(BRUSS_GRID_SIZE=2000, T_END="1.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=200, METHOD="DOPRI54")),
+    "pthreads_bruss_D_n2000_dopri54_sl200": ("pthreads", "bruss2d-mix", "D",
+        dict(BRUSS_GRID_SIZE=2000, T_END="1.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=200, METHOD="DOPRI54")),
+}
+
+# DUMP trajectories (every attempted step, full vector) on tiny grids
+DUMP_CASES = [
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=4), "DOPRI54", "4.0", "1E-6", "1E-3"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=5), "VERNER65", "4.0", "1E-6", "1E-3"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=7), "DOPRI54", "1.0", "1E-8", "1E-2"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=6), "RKF23", "0.5", "1E-5", "1E-3"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=3), "DOPRI87", "2.0", "1E-9", "1E-3"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=4), "RK4", "0.2", "0.0", "1E-2"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=2), "HEUN2", "0.1", "0.0", "1E-2"),
+    ("bruss2d-mix", dict(BRUSS_GRID_SIZE=3), "SSPRK3", "0.1", "0.0", "1E-2"),
+    ("lv", dict(LV_COUNT=3), "DOPRI54", "20.0", "1E-8", "1E-3"),
+    ("lv", dict(LV_COUNT=2), "VERNER65", "10.0", "1E-6", "1E-2"),
+]
+for prob, d, meth, te, tol, h0 in DUMP_CASES:
+    tag = "n%d" % d.get("BRUSS_GRID_SIZE", d.get("LV_COUNT"))
+    name = "dump_%s_%s_%s" % ("bruss" if prob.startswith("bruss") else "lv", tag, meth.lower())
+    defs = dict(d, T_END=te, TOL=tol, H_INIT=h0, STEP_LIMIT=0, METHOD=meth, DUMP=1)
+    CONFIGS[name] = ("seq", prob, "D", defs)
+
+
+def build_one(ref, name, kind, prob, impl, defs):
+    out = os.path.join(OUT, name)
+    srcs = [os.path.join(ref, "src", f) for f in ("main.c", "tools.c", "methods.c", "threads.c")]
+    srcs += [os.path.join(ref, "src", "problems", prob + ".c"),
+             os.path.join(ref, "src", "impl", kind, impl + ".c")]
+    cmd = ["gcc", "-I" + os.path.join(ref, "include"), "-D_REENTRANT", "-O3", "-g", "-D_GNU_SOURCE"]
+    cmd += ["-D%s=%s" % (k, v) for k, v in defs.items()]
+    cmd += ["-o", out] + srcs + ["-lpthread", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build of %s failed:\n%s" % (name, r.stderr[-2000:]))
+    return name
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", default=None, help="comma-separated config names")
+    a = ap.parse_args()
+    if not os.path.isdir(os.path.join(a.ref, "src")):
+        print("reference tree not present; oracle/_ref not built")
+        return 0
+    os.makedirs(OUT, exist_ok=True)
+    names = list(CONFIGS) if not a.only else a.only.split(",")
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        futs = [ex.submit(build_one, a.ref, n, *CONFIGS[n]) for n in names]
+        for f in futs:
+            f.result()
+    print("built %d reference binaries into %s" % (len(names), OUT))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

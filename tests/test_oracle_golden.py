@@ -1,0 +1,48 @@
+"""The oracle against the reference's DUMP trajectories (every attempted step,
+full vector, bitwise) on tiny grids: tests/golden/dump_*.npz, written by
+tests/golden/make_golden.py from the compiled reference."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+INDEX = json.load(open(os.path.join(GOLD, "dump_index.json")))
+
+
+def load_case(name):
+    meta = INDEX[name]
+    d = meta["defines"]
+    prob = "bruss2d" if meta["problem"].startswith("bruss") else "lv"
+    grid = int(d.get("BRUSS_GRID_SIZE", d.get("LV_COUNT", 0)))
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    return meta, prob, grid, d, z
+
+
+def parse_dump(path):
+    rows = [[float(x) for x in line.split("\t")] for line in open(path)]
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("name", sorted(INDEX))
+@pytest.mark.parametrize("impl", ["D", "PipeD"])
+def test_dump_trajectory_bitwise(orc, tmp_path, name, impl):
+    meta, prob, grid, d, z = load_case(name)
+    p = orc.make_ode(prob, grid)
+    y0 = orc.ode_start(p)
+    m = orc.method(d["METHOD"])
+    tol = float(d["TOL"]) if m.has_emb else 0.0
+    if impl == "PipeD" and p.n // orc.lib().orc_acc_dist(p) < m.s:
+        pytest.skip("grid too small for the block pipeline (n/B < s)")
+    dump = str(tmp_path / "DUMP")
+    y, st, _ = orc.solve(p, m, 0.0, float(d["T_END"]), y0, tol, h0=float(d["H_INIT"]),
+                         step_limit=0, impl=impl, dump_path=dump)
+    got = parse_dump(dump)
+    assert got.shape[0] == z["t"].shape[0]
+    assert np.array_equal(got[:, 0], z["t"])
+    assert np.array_equal(got[:, 1], z["h"])
+    assert np.array_equal(got[:, 2:], z["Y"])
+    assert (st["acc"], st["rej"]) == (meta["acc"], meta["rej"])
+    assert orc.footprint(y) == meta["footprint"]
