@@ -1,0 +1,146 @@
+/*
+ * rkx -- MI355X-native embedded Runge-Kutta solver: the C ABI.
+ *
+ * Plain C types only (pointers, sizes, doubles); no HIP or torch types cross
+ * this boundary (a stream is passed as void*).  Device pointers are marked
+ * "_dev"; everything else lives in host memory.  Every entry point returns 0
+ * on success or a negative RKX_E* code; rkx_last_error() gives the message
+ * of the calling thread's last failure.
+ *
+ * The reference (UBT-AI2/rk) binds its three plugin interfaces at compile and
+ * link time: the problem (include/ode.h:29-36), the method table
+ * (include/methods.h:92-93) and the solver (include/solver.h:47).  A GPU
+ * kernel cannot call host ode_eval_* callbacks, so the problem crosses this
+ * ABI as a descriptor (rkx_ode) naming one of the device right-hand sides,
+ * and the method crosses it as data with the layout of alloc_emb_rk_method
+ * (include/methods.h:31-42).  The link-compatible solver() of solver.h:47 is
+ * provided on top of this ABI by rk_amd/csrc/dropin/solver_hip.c.
+ */
+#ifndef RKX_H_
+#define RKX_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RKX_MAX_STAGES 16
+
+#define RKX_OK 0
+#define RKX_EINVAL (-1)        /* bad argument / unsupported combination */
+#define RKX_ENOMEM (-2)        /* device or host allocation failed */
+#define RKX_EDEVICE (-3)       /* HIP runtime error */
+#define RKX_ELIMIT (-4)        /* request exceeds a documented limit */
+
+/* ---- problems (replaces include/ode.h:29-36) ----------------------------- */
+enum {
+  RKX_BRUSS2D = 1,   /* src/problems/bruss2d-mix.c: 2N*N comps, (u,v) interleaved, d = 2N */
+  RKX_LV = 2,        /* src/problems/lv.c: pairs (x,y), d = 1 */
+  RKX_LV_CHAIN = 3   /* diffusive periodic LV chain (no reference counterpart), d = 2 */
+};
+
+typedef struct rkx_ode {
+  int kind;
+  int n;                 /* ode_size (ode.h:29) */
+  int grid;              /* BRUSS2D: N; LV: pairs; LV_CHAIN: cells */
+  double alpha;          /* BRUSS2D diffusion (config.h:34-36 BRUSS_ALPHA) */
+  double c0, d0, d1, c1; /* LV: x' = c0 x - d0 x y, y' = d1 x y - c1 y (lv.c:31-34) */
+  double diff;           /* LV_CHAIN diffusion coefficient */
+} rkx_ode;
+
+/* Fill a descriptor; replaces the compile-time BRUSS_GRID_SIZE/BRUSS_ALPHA
+ * (include/config.h:30-36) and LV_COUNT (config.h:42-44). */
+int rkx_ode_bruss2d(rkx_ode *p, int N, double alpha);
+int rkx_ode_lv(rkx_ode *p, int pairs, double c0, double d0, double d1, double c1);
+int rkx_ode_lv_chain(rkx_ode *p, int cells, double c0, double d0, double d1, double c1,
+                     double diff);
+
+/* ode_acc_dist() (ode.h:36; bruss2d-mix.c:718-721, lv.c:127-130) */
+int rkx_ode_acc_dist(const rkx_ode *p);
+
+/* ode_start(t, y0) (ode.h:33; bruss2d-mix.c:35-60, lv.c:43-63), host memory. */
+int rkx_ode_start(const rkx_ode *p, double t, double *y0);
+
+/* Start vector of a seeded batch instance: the BRUSS2D start vector plus
+ * amp*U[-1,1) noise on u from a counter-based hash of (seed, instance, index);
+ * LV chain: x = 0.2 + 0.1*U[0,1), y = 1.1 + 0.1*U[0,1).  No reference
+ * counterpart (the reference has no noise, bruss2d-mix.c:51-59). */
+int rkx_ode_start_seeded(const rkx_ode *p, unsigned long long seed, int instance,
+                         double amp, double *y0);
+
+/* ode_eval_rng(i, j, t, y, f) (ode.h:35) on the device: writes f_dev[i..j]
+ * inclusive with absolute indexing, reading y_dev.  The values equal the
+ * reference's ode_eval_comp bitwise. */
+int rkx_eval_rng_device(const rkx_ode *p, int i, int j, double t, const double *y_dev,
+                        double *f_dev, void *stream);
+
+/* ---- methods (replaces rk_method_t, include/methods.h:92-118) ------------ */
+typedef struct rkx_method {
+  char name[32];
+  int s, ord, has_emb;   /* has_emb = HAS_EMB_SOL(M) (methods.h:95-118) */
+  double A[RKX_MAX_STAGES][RKX_MAX_STAGES];
+  double b[RKX_MAX_STAGES], b_hat[RKX_MAX_STAGES], c[RKX_MAX_STAGES];
+} rkx_method;
+
+/* HEUN2, RK4, SSPRK3, RKF23, DOPRI54, VERNER65, DOPRI87 (src/methods.c:62-724) */
+int rkx_method_get(const char *name, rkx_method *m);
+
+/* ---- single solve (replaces solver(), include/solver.h:47) --------------- */
+enum {
+  RKX_IMPL_D = 0,        /* unfused stage kernels, full storage (src/impl/seq/D.c) */
+  RKX_IMPL_PIPED = 1,    /* stage-fused sweep, full storage of the step's output pair */
+  RKX_IMPL_PIPEDLS = 2   /* stage-fused sweep, low storage: two vectors of n */
+};
+
+enum {
+  RKX_CTRL_DEVICE = 0,   /* accept/reject and the new h on the device (fast) */
+  RKX_CTRL_HOST = 1      /* new h on the host with the C library's pow (bitwise
+                            reference step control; one host round trip per step) */
+};
+
+typedef struct rkx_opts {
+  double h0;             /* > 0: H_INIT; else initial_stepsize() (src/tools.c:25-36) */
+  int step_limit;        /* STEP_LIMIT (config.h:70-72); <= 0: unlimited */
+  int impl;              /* RKX_IMPL_* */
+  int ctrl;              /* RKX_CTRL_* */
+  int tile_rows;         /* sweep band height in grid rows (0: tuned default) */
+  int tile_waves;        /* sweep strip width in wavefronts (0: tuned default) */
+  double *trace;         /* optional host buffer, 3 doubles (t, h, err) per attempt */
+  int trace_cap;         /* attempts the trace buffer holds */
+  void *stream;          /* hipStream_t to run on; NULL: the library's own stream */
+} rkx_opts;
+
+typedef struct rkx_stats {
+  int steps_acc, steps_rej;   /* print_statistics (src/tools.c:62-79) */
+  double t, h;                /* final t and h */
+  double kernel_time;         /* seconds over the step loop (tools.c:77-78) */
+  size_t resident_bytes;      /* device bytes held by the solve */
+} rkx_stats;
+
+void rkx_opts_default(rkx_opts *o);
+
+/* Integrate y' = f(t, y) from t0 to te starting at y0_dev; the result goes to
+ * y_dev (may equal y0_dev).  Both are device pointers of p->n doubles.  tol <= 0
+ * means fixed step size (tools.h:223-224, 258-263). */
+int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
+              const double *y0_dev, double *y_dev, double tol, const rkx_opts *o,
+              rkx_stats *st);
+
+/* The same with host buffers (copies in and out; the PCIe copies are not
+ * inside kernel_time). */
+int rkx_solve_host(const rkx_ode *p, const rkx_method *m, double t0, double te,
+                   const double *y0, double *y, double tol, const rkx_opts *o,
+                   rkx_stats *st);
+
+/* ---- runtime -------------------------------------------------------------- */
+const char *rkx_last_error(void);
+int rkx_device_sync(void);
+/* Device bytes rkx_solve would hold for this problem/method/impl. */
+size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RKX_H_ */

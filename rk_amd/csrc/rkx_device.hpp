@@ -1,0 +1,215 @@
+// Device-side building blocks: right-hand sides, step control, reductions.
+//
+// Every floating-point expression keeps the reference's operand order and is
+// compiled with -ffp-contract=off, so results equal the reference bitwise.
+// A fused multiply-add is written explicitly only where the product is exact
+// (a power-of-two scale: 2*y, 4*y), where fma(a, b, c) == c + a*b bitwise.
+#pragma once
+
+#include "rkx_internal.hpp"
+
+namespace rkx {
+
+// ---- BRUSS2D-MIX (src/problems/bruss2d-mix.c:66-206) ------------------------
+// Laplacian numerator of one component at grid point (k, j) given its
+// neighbours.  vN/vS already carry the Neumann mirror on the top/bottom rows
+// (the reference's 2.0*y[i+-N2] equals y[i+N2] + y[i+N2] exactly), so only
+// the column boundaries need their own operand order:
+//   interior column:  ((vN + vS) + vW) + vE - 4 v
+//   j == 0:           (vN + vS) + 2 vE - 4 v
+//   j == N-1:         (vN + vS) + 2 vW - 4 v
+__device__ __forceinline__ double bruss_lap_col(double vN, double vS, double vW, double vE,
+                                                double v, int j, int N)
+{
+  const double t1 = vN + vS;
+  double t2;
+  if (j == 0)
+    t2 = __builtin_fma(2.0, vE, t1);
+  else if (j == N - 1)
+    t2 = __builtin_fma(2.0, vW, t1);
+  else
+    t2 = (t1 + vW) + vE;
+  return __builtin_fma(-4.0, v, t2);
+}
+
+// f_u = ((1 + u*u*v) - 4.4 u) + coef*lap_u ;  f_v = (3.4 u - u*u*v) + coef*lap_v
+__device__ __forceinline__ void bruss_react(double u, double v, double lu, double lv,
+                                            double coef, double &fu, double &fv)
+{
+  const double uuv = (u * u) * v;
+  fu = ((1.0 + uuv) - 4.4 * u) + coef * lu;
+  fv = (3.4 * u - uuv) + coef * lv;
+}
+
+// RHS of grid point cell = k*N + j reading a full state vector y (D form).
+__device__ __forceinline__ void bruss_cell(const dev_ode &p, const double *__restrict__ y,
+                                           int cell, double &fu, double &fv)
+{
+  const int N = p.N;
+  const int N2 = N + N;
+  const int k = cell / N;
+  const int j = cell - k * N;
+  const int i = k * N2 + 2 * j;
+  const int iN = (k == 0) ? i + N2 : i - N2;
+  const int iS = (k == N - 1) ? i - N2 : i + N2;
+  const double2 c = *reinterpret_cast<const double2 *>(y + i);
+  const double2 nN = *reinterpret_cast<const double2 *>(y + iN);
+  const double2 nS = *reinterpret_cast<const double2 *>(y + iS);
+  const double2 nW = (j > 0) ? *reinterpret_cast<const double2 *>(y + i - 2) : c;
+  const double2 nE = (j < N - 1) ? *reinterpret_cast<const double2 *>(y + i + 2) : c;
+  const double lu = bruss_lap_col(nN.x, nS.x, nW.x, nE.x, c.x, j, N);
+  const double lv = bruss_lap_col(nN.y, nS.y, nW.y, nE.y, c.y, j, N);
+  bruss_react(c.x, c.y, lu, lv, p.coef, fu, fv);
+}
+
+// ---- LV pair (src/problems/lv.c:69-77) ---------------------------------------
+__device__ __forceinline__ void lv_pair(const dev_ode &p, double x, double y, double &fx,
+                                        double &fy)
+{
+  fx = p.c0 * x - p.d0 * x * y;
+  fy = p.d1 * x * y - p.c1 * y;
+}
+
+// ---- LV chain (periodic, d = 2; no reference counterpart) -------------------
+//   x' = (c0 x - d0 x y) + D ((x_l + x_r) - 2x), likewise y.
+__device__ __forceinline__ void chain_cell(const dev_ode &p, const double *__restrict__ y,
+                                           int cell, double &fx, double &fy)
+{
+  const int l = (cell == 0) ? p.cells - 1 : cell - 1;
+  const int r = (cell == p.cells - 1) ? 0 : cell + 1;
+  const double2 c = *reinterpret_cast<const double2 *>(y + 2 * cell);
+  const double2 a = *reinterpret_cast<const double2 *>(y + 2 * l);
+  const double2 b = *reinterpret_cast<const double2 *>(y + 2 * r);
+  double gx, gy;
+  lv_pair(p, c.x, c.y, gx, gy);
+  fx = gx + p.diff * __builtin_fma(-2.0, c.x, a.x + b.x);
+  fy = gy + p.diff * __builtin_fma(-2.0, c.y, a.y + b.y);
+}
+
+__device__ __forceinline__ void rhs_cell(const dev_ode &p, const double *__restrict__ y,
+                                         int cell, double &f0, double &f1)
+{
+  if (p.kind == RKX_BRUSS2D) {
+    bruss_cell(p, y, cell, f0, f1);
+  } else if (p.kind == RKX_LV) {
+    const double2 c = *reinterpret_cast<const double2 *>(y + 2 * cell);
+    lv_pair(p, c.x, c.y, f0, f1);
+  } else {
+    chain_cell(p, y, cell, f0, f1);
+  }
+}
+
+// ---- error norm (include/tools.h:280-288) ------------------------------------
+// Per component q = |e| / max(|y_new|, |y_old|); a zero divisor raises the
+// 1.7e308 sentinel, a NaN quotient is skipped (fmax semantics).
+__device__ __forceinline__ void err_term(double e, double yn, double yo, double &qmax,
+                                         unsigned int &zero)
+{
+  const double d = fmax(fabs(yn), fabs(yo));
+  if (d == 0.0)
+    zero = 1u;
+  else
+    qmax = fmax(qmax, fabs(e) / d);
+}
+
+// ---- step control (include/tools.h:217-278, 191-198) ------------------------
+__device__ __forceinline__ double dev_h_new(double h, double err, double tol, int ord)
+{
+  return h * fmin(2.0, fmax(0.5, 0.9 * pow(tol / err, 1.0 / (ord + 1.0))));
+}
+
+// Runs in exactly one thread after every workgroup has folded its maximum into
+// st->err_bits / st->zero_div.  Resets the accumulators for the next step.
+__device__ inline void dev_control(state *st)
+{
+  const unsigned long long bits = atomicOr(&st->err_bits, 0ull);
+  const unsigned int zero = atomicOr(&st->zero_div, 0u);
+  const double q = __longlong_as_double(static_cast<long long>(bits));
+  const double err = zero ? fmax(1.7E308, q) : q;
+  st->err_bits = 0ull;
+  st->zero_div = 0u;
+  st->blocks_done = 0u;
+  st->err = err;
+  if (st->host_ctrl)
+    return;
+  double t = st->t, h = st->h;
+  const double tol = st->tol, te = st->te;
+  const int attempt = st->acc + st->rej;
+  if (st->trace && attempt < st->trace_cap) {
+    st->trace[3 * attempt + 0] = t;
+    st->trace[3 * attempt + 1] = h;
+    st->trace[3 * attempt + 2] = err;
+  }
+  int accept;
+  if (tol <= 0.0) {
+    t += h;
+    accept = 1;
+  } else if (err <= tol) {
+    t += h;
+    h = dev_h_new(h, err, tol, st->ord);
+    accept = 1;
+  } else {
+    h = dev_h_new(h, err, tol, st->ord);
+    accept = 0;
+  }
+  if (accept) {
+    st->acc += 1;
+    st->cur ^= 1;
+  } else {
+    st->rej += 1;
+  }
+  h = fmin(h, te - t);
+  st->t = t;
+  st->h = h;
+  premult(*st->meth, st->s, h, *st->coef);
+  const int attempts = st->acc + st->rej;
+  st->done = !((t < te) && (st->step_limit <= 0 || attempts < st->step_limit));
+}
+
+// ---- reductions -----------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+  for (int off = 32; off > 0; off >>= 1)
+    v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Folds each thread's (qmax, zero) into the step accumulators and runs the
+// control in the last workgroup to finish.  Must be called by all threads of
+// the block.  red must hold blockDim.x/64 doubles of LDS.
+__device__ inline void block_finish(state *st, double qmax, unsigned int zero, double *red,
+                                    unsigned int *zred)
+{
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  qmax = wave_max(qmax);
+  const unsigned long long zb = __ballot(zero != 0u);
+  if (lane == 0) {
+    red[wid] = qmax;
+    zred[wid] = zb ? 1u : 0u;
+  }
+  __syncthreads();
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    unsigned int z = zred[0];
+    for (int w = 1; w < nw; ++w) {
+      m = fmax(m, red[w]);
+      z |= zred[w];
+    }
+    atomicMax(&st->err_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+    if (z)
+      atomicOr(&st->zero_div, 1u);
+    __threadfence();
+    const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
+    is_last = (prev == gridDim.x * gridDim.y * gridDim.z - 1u);
+  }
+  __syncthreads();
+  if (is_last && threadIdx.x == 0) {
+    __threadfence();
+    dev_control(st);
+  }
+}
+
+}  // namespace rkx

@@ -1,0 +1,574 @@
+// rkx host runtime: the C ABI of include/rkx.h over HIP.
+//
+// The step loop of the reference's solver() (src/impl/seq/D.c:58-78) runs on
+// the device: every step's kernels read t, h and the current-buffer index from
+// a device-resident state and the last workgroup of the step's final kernel
+// decides accept/reject and the next h.  The host only enqueues chunks of
+// steps, captured once as a hipGraph, and polls the state between chunks;
+// kernels of a chunk that run past the end of the integration return at once.
+#include "rkx_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace rkx {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int fail(int code, const std::string &msg)
+{
+  g_last_error = msg;
+  return code;
+}
+
+dev_ode make_dev_ode(const rkx_ode &p)
+{
+  dev_ode d{};
+  d.kind = p.kind;
+  d.n = p.n;
+  d.cells = p.n / 2;
+  d.N = (p.kind == RKX_BRUSS2D) ? p.grid : 0;
+  if (p.kind == RKX_BRUSS2D) {
+    const double N1 = static_cast<double>(p.grid) - 1.0;
+    d.coef = p.alpha * N1 * N1;   // ((alpha*N1)*N1), bruss2d-mix.c:77
+  }
+  d.c0 = p.c0;
+  d.d0 = p.d0;
+  d.d1 = p.d1;
+  d.c1 = p.c1;
+  d.diff = p.diff;
+  return d;
+}
+
+// h_new_acc/h_new_rej and step_control (include/tools.h:221-278) with the C
+// library's pow, then the for-increment h = fmin(h, te - t) (tools.h:191-198).
+void host_control(state &st)
+{
+  const double err = st.err;
+  const int attempt = st.acc + st.rej;
+  (void) attempt;
+  int accept;
+  if (st.tol <= 0.0) {
+    st.t += st.h;
+    accept = 1;
+  } else if (err <= st.tol) {
+    st.t += st.h;
+    st.h = st.h * std::fmin(2.0, std::fmax(0.5, 0.9 * std::pow(st.tol / err, 1.0 / (st.ord + 1.0))));
+    accept = 1;
+  } else {
+    st.h = st.h * std::fmin(2.0, std::fmax(0.5, 0.9 * std::pow(st.tol / err, 1.0 / (st.ord + 1.0))));
+    accept = 0;
+  }
+  if (accept) {
+    st.acc += 1;
+    st.cur ^= 1;
+  } else {
+    st.rej += 1;
+  }
+  st.h = std::fmin(st.h, st.te - st.t);
+  st.done = !((st.t < st.te) && (st.step_limit <= 0 || st.acc + st.rej < st.step_limit));
+}
+
+namespace {
+
+hipStream_t g_stream = nullptr;
+std::mutex g_init_mu;
+
+int default_stream(hipStream_t *out)
+{
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (!g_stream) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      return fail(RKX_EDEVICE, "no HIP device available");
+    RKX_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  }
+  *out = g_stream;
+  return RKX_OK;
+}
+
+int check_ode(const rkx_ode *p)
+{
+  if (!p)
+    return fail(RKX_EINVAL, "null problem descriptor");
+  switch (p->kind) {
+  case RKX_BRUSS2D:
+    if (p->grid < 2 || p->n != 2 * p->grid * p->grid)
+      return fail(RKX_EINVAL, "BRUSS2D needs N >= 2 and n = 2*N*N");
+    if (p->grid > 32768)
+      return fail(RKX_ELIMIT, "BRUSS2D grid above 32768 (n must fit in int)");
+    break;
+  case RKX_LV:
+    if (p->grid < 1 || p->n != 2 * p->grid)
+      return fail(RKX_EINVAL, "LV needs pairs >= 1 and n = 2*pairs");
+    break;
+  case RKX_LV_CHAIN:
+    if (p->grid < 3 || p->n != 2 * p->grid)
+      return fail(RKX_EINVAL, "LV chain needs cells >= 3 and n = 2*cells");
+    break;
+  default:
+    return fail(RKX_EINVAL, "unknown problem kind");
+  }
+  return RKX_OK;
+}
+
+int check_method(const rkx_method *m)
+{
+  if (!m || m->s < 2 || m->s > RKX_MAX_STAGES)
+    return fail(RKX_EINVAL, "method needs 2 <= s <= 16 stages");
+  for (int i = 0; i < m->s; ++i)
+    for (int j = i; j < m->s; ++j)
+      if (m->A[i][j] != 0.0)
+        return fail(RKX_EINVAL, "method is not explicit (A[i][j] != 0 for j >= i)");
+  return RKX_OK;
+}
+
+void fill_dev_method(const rkx_method &m, dev_method &d)
+{
+  std::memset(&d, 0, sizeof(d));
+  for (int i = 0; i < m.s; ++i) {
+    for (int j = 0; j < i; ++j)
+      d.A[i * RKX_MAX_STAGES + j] = m.A[i][j];
+    d.b[i] = m.b[i];
+    d.e[i] = m.b[i] - m.b_hat[i];      // seq/D.c:40-41
+    d.c[i] = m.c[i];
+  }
+}
+
+// initial_stepsize (src/tools.c:25-36)
+double initial_h(double H, double tol, int limit, double h0)
+{
+  if (h0 > 0.0)
+    return h0;
+  return tol > 0.0 ? std::fmin(10.0 * tol, H / static_cast<double>(limit))
+                   : H / static_cast<double>(limit);
+}
+
+// All device memory of one solve, released on scope exit.
+struct solve_mem {
+  std::vector<void *> ptrs;
+  ~solve_mem()
+  {
+    for (void *p : ptrs)
+      (void) hipFree(p);
+  }
+  int alloc(void **out, size_t bytes)
+  {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void) hipGetLastError();
+      return fail(RKX_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    ptrs.push_back(p);
+    *out = p;
+    return RKX_OK;
+  }
+};
+
+struct graph_holder {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  ~graph_holder()
+  {
+    if (ge)
+      (void) hipGraphExecDestroy(ge);
+    if (g)
+      (void) hipGraphDestroy(g);
+  }
+};
+
+}  // namespace
+
+}  // namespace rkx
+
+using namespace rkx;
+
+extern "C" {
+
+const char *rkx_last_error(void) { return g_last_error.c_str(); }
+
+int rkx_device_sync(void)
+{
+  RKX_HIP(hipDeviceSynchronize());
+  return RKX_OK;
+}
+
+void rkx_opts_default(rkx_opts *o)
+{
+  if (!o)
+    return;
+  std::memset(o, 0, sizeof(*o));
+  o->impl = RKX_IMPL_PIPEDLS;
+  o->ctrl = RKX_CTRL_DEVICE;
+}
+
+int rkx_ode_bruss2d(rkx_ode *p, int N, double alpha)
+{
+  if (!p || N < 2 || N > 32768)
+    return fail(RKX_EINVAL, "rkx_ode_bruss2d: need 2 <= N <= 32768");
+  std::memset(p, 0, sizeof(*p));
+  p->kind = RKX_BRUSS2D;
+  p->grid = N;
+  p->n = 2 * N * N;
+  p->alpha = alpha;
+  return RKX_OK;
+}
+
+int rkx_ode_lv(rkx_ode *p, int pairs, double c0, double d0, double d1, double c1)
+{
+  if (!p || pairs < 1 || pairs > (1 << 29))
+    return fail(RKX_EINVAL, "rkx_ode_lv: bad pair count");
+  std::memset(p, 0, sizeof(*p));
+  p->kind = RKX_LV;
+  p->grid = pairs;
+  p->n = 2 * pairs;
+  p->c0 = c0;
+  p->d0 = d0;
+  p->d1 = d1;
+  p->c1 = c1;
+  return RKX_OK;
+}
+
+int rkx_ode_lv_chain(rkx_ode *p, int cells, double c0, double d0, double d1, double c1,
+                     double diff)
+{
+  if (!p || cells < 3 || cells > (1 << 29))
+    return fail(RKX_EINVAL, "rkx_ode_lv_chain: need 3 <= cells");
+  std::memset(p, 0, sizeof(*p));
+  p->kind = RKX_LV_CHAIN;
+  p->grid = cells;
+  p->n = 2 * cells;
+  p->c0 = c0;
+  p->d0 = d0;
+  p->d1 = d1;
+  p->c1 = c1;
+  p->diff = diff;
+  return RKX_OK;
+}
+
+int rkx_ode_acc_dist(const rkx_ode *p)
+{
+  if (check_ode(p))
+    return RKX_EINVAL;
+  switch (p->kind) {
+  case RKX_BRUSS2D: return 2 * p->grid;
+  case RKX_LV: return 1;
+  default: return 2;
+  }
+}
+
+int rkx_ode_start(const rkx_ode *p, double t, double *y0)
+{
+  (void) t;
+  if (int rc = check_ode(p))
+    return rc;
+  if (!y0)
+    return fail(RKX_EINVAL, "rkx_ode_start: null y0");
+  if (p->kind == RKX_BRUSS2D) {
+    const int N = p->grid;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        y0[2 * N * i + 2 * j] = 0.5 + static_cast<double>(j) / static_cast<double>(N - 1);
+        y0[2 * N * i + 2 * j + 1] =
+            1.0 + (5.0 * static_cast<double>(i)) / static_cast<double>(N - 1);
+      }
+  } else {
+    for (int i = 0; i < p->n; i += 2) {   // lv.c:58-62
+      y0[i] = 0.2;
+      y0[i + 1] = 1.1;
+    }
+  }
+  return RKX_OK;
+}
+
+// splitmix64 finaliser over (seed, instance, index): a counter-based generator,
+// so every value is reproducible independently of generation order.
+static inline unsigned long long rkx_mix(unsigned long long z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static inline double rkx_unit(unsigned long long seed, int inst, long long idx)
+{
+  const unsigned long long k =
+      rkx_mix(seed ^ rkx_mix(static_cast<unsigned long long>(inst) * 0x100000001B3ull) ^
+              rkx_mix(static_cast<unsigned long long>(idx) + 0x632BE59BD9B4E019ull));
+  return static_cast<double>(k >> 11) * 0x1.0p-53;   // [0, 1)
+}
+
+int rkx_ode_start_seeded(const rkx_ode *p, unsigned long long seed, int instance, double amp,
+                         double *y0)
+{
+  if (int rc = rkx_ode_start(p, 0.0, y0))
+    return rc;
+  if (p->kind == RKX_BRUSS2D) {
+    const int N = p->grid;
+    for (long long c = 0; c < static_cast<long long>(N) * N; ++c)
+      y0[2 * c] = y0[2 * c] + amp * (2.0 * rkx_unit(seed, instance, c) - 1.0);
+  } else {
+    for (int c = 0; c < p->n / 2; ++c) {
+      y0[2 * c] = 0.2 + 0.1 * rkx_unit(seed, instance, 2 * c);
+      y0[2 * c + 1] = 1.1 + 0.1 * rkx_unit(seed, instance, 2 * c + 1);
+    }
+  }
+  return RKX_OK;
+}
+
+int rkx_eval_rng_device(const rkx_ode *p, int i, int j, double t, const double *y_dev,
+                        double *f_dev, void *stream)
+{
+  (void) t;
+  if (int rc = check_ode(p))
+    return rc;
+  if (i < 0 || j >= p->n || j < i || !y_dev || !f_dev)
+    return fail(RKX_EINVAL, "rkx_eval_rng_device: bad range or pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!s)
+    if (int rc = default_stream(&s))
+      return rc;
+  const dev_ode d = make_dev_ode(*p);
+  RKX_HIP(launch_eval_rng(d, i, j, y_dev, f_dev, s));
+  RKX_HIP(hipStreamSynchronize(s));
+  return RKX_OK;
+}
+
+size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
+{
+  if (check_ode(p) || check_method(m))
+    return 0;
+  const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
+  const size_t fixed = sizeof(state) + sizeof(dev_method) + sizeof(dev_coef);
+  if (impl == RKX_IMPL_D)
+    return (2 + (m->s - 1) + 2) * vec + fixed;   // y x2, w_1..w_{s-1}, dy, err
+  return 2 * vec + fixed;                        // y x2
+}
+
+int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
+              const double *y0_dev, double *y_dev, double tol, const rkx_opts *o_in,
+              rkx_stats *st_out)
+{
+  if (int rc = check_ode(p))
+    return rc;
+  if (int rc = check_method(m))
+    return rc;
+  if (!y0_dev || !y_dev)
+    return fail(RKX_EINVAL, "rkx_solve: null vector");
+  rkx_opts o;
+  if (o_in)
+    o = *o_in;
+  else
+    rkx_opts_default(&o);
+  if (o.impl < RKX_IMPL_D || o.impl > RKX_IMPL_PIPEDLS)
+    return fail(RKX_EINVAL, "rkx_solve: unknown impl");
+  if (o.trace_cap < 0 || (o.trace_cap > 0 && !o.trace))
+    return fail(RKX_EINVAL, "rkx_solve: trace_cap without a trace buffer");
+  hipStream_t stream = static_cast<hipStream_t>(o.stream);
+  if (!stream)
+    if (int rc = default_stream(&stream))
+      return rc;
+
+  const dev_ode dp = make_dev_ode(*p);
+  const int s = m->s;
+  int impl = o.impl;
+  if (impl != RKX_IMPL_D && !sweep_supported(dp, *m))
+    impl = RKX_IMPL_D;   // the fused sweep covers BRUSS2D; others use the stage kernels
+  const size_t n = static_cast<size_t>(p->n);
+  const size_t vec = n * sizeof(double);
+
+  solve_mem mem;
+  double *ybuf[2] = {nullptr, nullptr};
+  double *w = nullptr, *dy = nullptr, *err = nullptr, *trace_dev = nullptr;
+  state *st_dev = nullptr;
+  dev_method *dm_dev = nullptr;
+  dev_coef *cf_dev = nullptr;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&st_dev), sizeof(state)))
+    return rc;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&dm_dev), sizeof(dev_method)))
+    return rc;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&cf_dev), sizeof(dev_coef)))
+    return rc;
+  for (int b = 0; b < 2; ++b)
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&ybuf[b]), vec))
+      return rc;
+  if (impl == RKX_IMPL_D) {
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&w), vec * (s - 1)))
+      return rc;
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&dy), vec))
+      return rc;
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&err), vec))
+      return rc;
+  }
+  if (o.trace_cap > 0)
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&trace_dev),
+                           sizeof(double) * 3 * static_cast<size_t>(o.trace_cap)))
+      return rc;
+
+  dev_method dm;
+  fill_dev_method(*m, dm);
+  state hs{};
+  hs.t = t0;
+  hs.te = te;
+  hs.tol = tol;
+  hs.h = initial_h(te - t0, tol, o.step_limit, o.h0);
+  hs.step_limit = o.step_limit;
+  hs.ord = m->ord;
+  hs.host_ctrl = (o.ctrl == RKX_CTRL_HOST) ? 1 : 0;
+  hs.trace = trace_dev;
+  hs.trace_cap = o.trace_cap;
+  hs.s = s;
+  hs.meth = dm_dev;
+  hs.coef = cf_dev;
+  dev_coef hc{};
+  premult(dm, s, hs.h, hc);
+  hs.cur = 0;
+  hs.done = !(t0 < te);
+
+  RKX_HIP(hipMemcpyAsync(dm_dev, &dm, sizeof(dm), hipMemcpyHostToDevice, stream));
+  RKX_HIP(hipMemcpyAsync(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice, stream));
+  RKX_HIP(hipMemcpyAsync(st_dev, &hs, sizeof(hs), hipMemcpyHostToDevice, stream));
+  RKX_HIP(hipMemcpyAsync(ybuf[0], y0_dev, vec, hipMemcpyDeviceToDevice, stream));
+
+  sweep_cfg cfg = sweep_default_cfg(dp, *m);
+  if (o.tile_rows > 0)
+    cfg.rows = o.tile_rows;
+  if (o.tile_waves > 0)
+    cfg.waves = o.tile_waves;
+
+  auto enqueue_step = [&]() -> hipError_t {
+    if (impl == RKX_IMPL_D)
+      return launch_d_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], w, dy, err, stream);
+    return launch_sweep_step(dp, *m, cf_dev, st_dev, ybuf[0], ybuf[1], cfg, stream);
+  };
+
+  state *hs_pinned = nullptr;   // [0], [1]: snapshots of two chunks in flight
+  RKX_HIP(hipHostMalloc(reinterpret_cast<void **>(&hs_pinned), 2 * sizeof(state),
+                        hipHostMallocDefault));
+  struct pinned_free {
+    state *p;
+    ~pinned_free() { (void) hipHostFree(p); }
+  } pf{hs_pinned};
+
+  RKX_HIP(hipStreamSynchronize(stream));
+  const auto t_start = std::chrono::steady_clock::now();
+
+  if (hs.done) {
+    hs_pinned[0] = hs;
+  } else if (o.ctrl == RKX_CTRL_HOST) {
+    // One host round trip per step: the device computes the error norm, the
+    // host runs the reference's step control with the C library's pow.
+    state cur = hs;
+    while (!cur.done) {
+      RKX_HIP(enqueue_step());
+      RKX_HIP(hipMemcpyAsync(hs_pinned, st_dev, sizeof(state), hipMemcpyDeviceToHost, stream));
+      RKX_HIP(hipStreamSynchronize(stream));
+      cur = hs_pinned[0];
+      const int attempt = cur.acc + cur.rej;
+      if (o.trace && attempt < o.trace_cap) {
+        o.trace[3 * attempt + 0] = cur.t;
+        o.trace[3 * attempt + 1] = cur.h;
+        o.trace[3 * attempt + 2] = cur.err;
+      }
+      host_control(cur);
+      premult(dm, s, cur.h, hc);
+      RKX_HIP(hipMemcpyAsync(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice, stream));
+      hs_pinned[0] = cur;
+      RKX_HIP(hipMemcpyAsync(st_dev, hs_pinned, sizeof(state), hipMemcpyHostToDevice, stream));
+    }
+    RKX_HIP(hipStreamSynchronize(stream));
+    hs_pinned[0] = cur;
+  } else {
+    // Chunks of steps captured once as a graph.  The host keeps two chunks in
+    // flight: it enqueues chunk k+1 (and a snapshot copy of the state) before
+    // waiting on the snapshot taken after chunk k.
+    const int chunk = (impl == RKX_IMPL_D) ? 16 : 32;
+    graph_holder gh;
+    RKX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipError_t cap_err = hipSuccess;
+    for (int k = 0; k < chunk && cap_err == hipSuccess; ++k)
+      cap_err = enqueue_step();
+    hipError_t end_err = hipStreamEndCapture(stream, &gh.g);
+    if (cap_err != hipSuccess)
+      return fail(RKX_EDEVICE, std::string("kernel launch during capture failed: ") +
+                                   hipGetErrorString(cap_err));
+    RKX_HIP(end_err);
+    RKX_HIP(hipGraphInstantiate(&gh.ge, gh.g, nullptr, nullptr, 0));
+    hipEvent_t ev[2];
+    RKX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    RKX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    struct ev_free {
+      hipEvent_t *e;
+      ~ev_free()
+      {
+        (void) hipEventDestroy(e[0]);
+        (void) hipEventDestroy(e[1]);
+      }
+    } evf{ev};
+    RKX_HIP(hipGraphLaunch(gh.ge, stream));
+    RKX_HIP(hipMemcpyAsync(&hs_pinned[0], st_dev, sizeof(state), hipMemcpyDeviceToHost, stream));
+    RKX_HIP(hipEventRecord(ev[0], stream));
+    for (int k = 0;; ++k) {
+      const int a = k & 1, b = a ^ 1;
+      RKX_HIP(hipGraphLaunch(gh.ge, stream));
+      RKX_HIP(hipMemcpyAsync(&hs_pinned[b], st_dev, sizeof(state), hipMemcpyDeviceToHost, stream));
+      RKX_HIP(hipEventRecord(ev[b], stream));
+      RKX_HIP(hipEventSynchronize(ev[a]));
+      if (hs_pinned[a].done)
+        break;
+    }
+    RKX_HIP(hipStreamSynchronize(stream));
+    RKX_HIP(hipMemcpy(&hs_pinned[0], st_dev, sizeof(state), hipMemcpyDeviceToHost));
+  }
+  const auto t_end = std::chrono::steady_clock::now();
+  const state fin = hs_pinned[0];
+  RKX_HIP(hipMemcpyAsync(y_dev, ybuf[fin.cur], vec, hipMemcpyDeviceToDevice, stream));
+  if (o.trace_cap > 0 && o.ctrl != RKX_CTRL_HOST) {
+    const int ntr = std::min(o.trace_cap, fin.acc + fin.rej);
+    if (ntr > 0)
+      RKX_HIP(hipMemcpyAsync(o.trace, trace_dev, sizeof(double) * 3 * ntr, hipMemcpyDeviceToHost,
+                             stream));
+  }
+  RKX_HIP(hipStreamSynchronize(stream));
+  if (st_out) {
+    st_out->steps_acc = fin.acc;
+    st_out->steps_rej = fin.rej;
+    st_out->t = fin.t;
+    st_out->h = fin.h;
+    st_out->kernel_time = std::chrono::duration<double>(t_end - t_start).count();
+    st_out->resident_bytes = rkx_solve_footprint(p, m, impl);
+  }
+  return RKX_OK;
+}
+
+int rkx_solve_host(const rkx_ode *p, const rkx_method *m, double t0, double te, const double *y0,
+                   double *y, double tol, const rkx_opts *o, rkx_stats *st)
+{
+  if (int rc = check_ode(p))
+    return rc;
+  if (!y0 || !y)
+    return fail(RKX_EINVAL, "rkx_solve_host: null vector");
+  const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
+  solve_mem mem;
+  double *d0 = nullptr, *d1 = nullptr;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&d0), vec))
+    return rc;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&d1), vec))
+    return rc;
+  RKX_HIP(hipMemcpy(d0, y0, vec, hipMemcpyHostToDevice));
+  if (int rc = rkx_solve(p, m, t0, te, d0, d1, tol, o, st))
+    return rc;
+  RKX_HIP(hipMemcpy(y, d1, vec, hipMemcpyDeviceToHost));
+  return RKX_OK;
+}
+
+}  // extern "C"

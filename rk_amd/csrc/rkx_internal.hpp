@@ -1,0 +1,108 @@
+// Internal declarations shared by the rkx host runtime and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/rkx.h"
+
+namespace rkx {
+
+int fail(int code, const std::string &msg);
+
+#define RKX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return ::rkx::fail(RKX_EDEVICE, std::string(#call " failed: ") +             \
+                                          hipGetErrorString(e_));                 \
+  } while (0)
+
+// Problem descriptor as the kernels see it (by value in kernel arguments).
+struct dev_ode {
+  int kind;
+  int n;        // components
+  int cells;    // n / 2: grid points (BRUSS2D), pairs (LV), cells (chain)
+  int N;        // BRUSS2D grid size
+  double coef;  // BRUSS2D (alpha*(N-1))*(N-1), the reference's alpha*N1*N1
+  double c0, d0, d1, c1;
+  double diff;
+};
+
+dev_ode make_dev_ode(const rkx_ode &p);
+
+// Method coefficients in device memory: A row-major [RKX_MAX_STAGES^2], then
+// b, e = b - b_hat (src/impl/seq/D.c:40-41), c.
+struct dev_method {
+  double A[RKX_MAX_STAGES * RKX_MAX_STAGES];
+  double b[RKX_MAX_STAGES];
+  double e[RKX_MAX_STAGES];
+  double c[RKX_MAX_STAGES];
+};
+
+// Premultiplied coefficients of the current step (premult, src/methods.c:43-58):
+// hA = h*A (j < i), hb = h*b, he = h*e.  Written by the step control together
+// with the new h; read as uniform values by every kernel of the next step.
+struct dev_coef {
+  double hA[RKX_MAX_STAGES * RKX_MAX_STAGES];
+  double hb[RKX_MAX_STAGES];
+  double he[RKX_MAX_STAGES];
+};
+
+// Integration state of one solve, resident in device memory.  Written only by
+// the last workgroup of a step (or by the host in host-control mode).
+struct alignas(64) state {
+  double t, h, te, tol;
+  double err;                        // error norm of the last attempt
+  unsigned long long err_bits;       // atomicMax accumulator (bits of a double >= 0)
+  unsigned int zero_div;             // some divisor max(|y|,|y_old|) was 0
+  unsigned int blocks_done;          // last-workgroup detection
+  int acc, rej;
+  int cur;                           // which of the two y buffers holds y
+  int done;                          // the step loop has ended
+  int step_limit;
+  int ord;
+  int host_ctrl;                     // 1: last workgroup only finalizes err
+  int trace_cap;
+  int s;                             // stages
+  double *trace;                     // device buffer, 3 doubles per attempt
+  const dev_method *meth;            // raw coefficients for the premultiplication
+  dev_coef *coef;                    // premultiplied coefficients of the next step
+};
+
+// premult of the raw table by h (identical on host and device)
+__host__ __device__ inline void premult(const dev_method &m, int s, double h, dev_coef &c)
+{
+  for (int i = 0; i < s; ++i) {
+    for (int j = 0; j < i; ++j)
+      c.hA[i * RKX_MAX_STAGES + j] = h * m.A[i * RKX_MAX_STAGES + j];
+    c.hb[i] = h * m.b[i];
+    c.he[i] = h * m.e[i];
+  }
+}
+
+// Host-side step control, the bitwise reference form (include/tools.h:221-278
+// and the for-increment of FOR_ALL_GRIDPOINTS, tools.h:191-198).
+void host_control(state &st);
+
+// Kernel launchers (rkx_kernels_*.hip).  All return hipError_t of the launch.
+hipError_t launch_eval_rng(const dev_ode &p, int i, int j, const double *y, double *f,
+                           hipStream_t s);
+hipError_t launch_d_step(const dev_ode &p, const dev_coef *cf, int s, state *st,
+                         double *y0, double *y1, double *w, double *dy, double *err,
+                         hipStream_t stream);
+
+struct sweep_cfg {
+  int waves;     // wavefronts per workgroup (strip width 64*waves lanes)
+  int rows;      // band height in grid rows
+};
+bool sweep_supported(const dev_ode &p, const rkx_method &m);
+hipError_t launch_sweep_step(const dev_ode &p, const rkx_method &m, const dev_coef *cf,
+                             state *st, double *y0, double *y1, const sweep_cfg &cfg,
+                             hipStream_t stream);
+sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m);
+
+}  // namespace rkx

@@ -1,0 +1,75 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol that
+include/rkx.h declares, and its host-side pieces (method tables, start
+vectors, descriptors) equal the oracle's bitwise.  No device calls."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(REPO, "include", "rkx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:int|void|size_t|char\s*\*|const char\s*\*)\s*\**\s*(rkx_\w+)\s*\(",
+                       src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_header():
+    import rk_amd
+    L = rk_amd.lib()
+    names = header_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(L, n), n
+    # and the Python declarations cover them all
+    from rk_amd._lib import SIGNATURES
+    assert sorted(n for n, _, _ in SIGNATURES) == names
+
+
+@pytest.mark.parametrize("name", ["HEUN2", "RK4", "SSPRK3", "RKF23", "DOPRI54", "VERNER65",
+                                  "DOPRI87"])
+def test_method_tables_match_oracle(orc, name):
+    import rk_amd
+    A, b, bh, c, s, o = rk_amd.method_table(name)
+    Ao, bo, bho, co, so, oo = orc.method_arrays(name)
+    assert (s, o) == (so, oo)
+    assert np.array_equal(A, Ao) and np.array_equal(b, bo)
+    assert np.array_equal(bh, bho) and np.array_equal(c, co)
+    assert rk_amd.method(name).has_emb == orc.method(name).has_emb
+
+
+@pytest.mark.parametrize("N", [2, 3, 17, 128])
+def test_bruss_start_matches_oracle(orc, N):
+    import rk_amd
+    p = rk_amd.Bruss2D(N)
+    assert p.ode_size == 2 * N * N and p.ode_acc_dist() == 2 * N
+    assert np.array_equal(p.ode_start(), orc.ode_start(orc.make_ode("bruss2d", N)))
+    # zero noise amplitude reproduces the reference start vector
+    assert np.array_equal(p.ode_start_seeded(7, 3, 0.0), p.ode_start())
+    a = p.ode_start_seeded(7, 3, 0.01)
+    assert np.abs(a - p.ode_start()).max() <= 0.01 and not np.array_equal(a, p.ode_start())
+    assert np.array_equal(a, p.ode_start_seeded(7, 3, 0.01))
+
+
+def test_lv_descriptors(orc):
+    import rk_amd
+    p = rk_amd.LV(5)
+    assert p.ode_size == 10 and p.ode_acc_dist() == 1
+    assert np.array_equal(p.ode_start(), orc.ode_start(orc.make_ode("lv", 5)))
+    q = rk_amd.LVChain(9)
+    assert q.ode_size == 18 and q.ode_acc_dist() == 2
+
+
+def test_bad_arguments_raise():
+    import rk_amd
+    with pytest.raises(rk_amd.RkxError):
+        rk_amd.Bruss2D(1)
+    with pytest.raises(rk_amd.RkxError):
+        rk_amd.method("NOPE")
+    with pytest.raises(rk_amd.RkxError):
+        rk_amd.LVChain(2)

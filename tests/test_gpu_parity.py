@@ -1,0 +1,185 @@
+"""GPU parity of the HIP path against the CPU oracle (seq D restated) and the
+reference's known answers.
+
+Bar: with host step control (the C library's pow, as in include/tools.h:226)
+every attempted step's (t, h, err) and the final vector equal the oracle's
+bitwise.  With device step control (the device pow) the accepted/rejected
+counts must be identical and the final vector within 1e-12 relative of the
+oracle (the only difference allowed is an ulp of h from pow).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "kat.json")))["cases"]
+GOLD = os.path.join(HERE, "golden")
+DUMPS = json.load(open(os.path.join(GOLD, "dump_index.json")))
+
+REL_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def rk():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import rk_amd
+    rk_amd.lib()
+    return rk_amd
+
+
+def dev(a):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    torch.cuda.synchronize()
+    return t
+
+
+def run_gpu(rk, prob, meth, t0, te, y0, tol, **kw):
+    import torch
+    y0d = dev(y0)
+    yd = torch.empty_like(y0d)
+    st, tr = rk.solve(prob, meth, t0, te, y0d, yd, tol, **kw)
+    torch.cuda.synchronize()
+    return yd.cpu().numpy(), st, tr
+
+
+def max_rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def gpu_problem(rk, name, grid, alpha=2e-3):
+    if name == "bruss2d":
+        return rk.Bruss2D(grid, alpha)
+    if name == "lv":
+        return rk.LV(grid)
+    return rk.LVChain(grid)
+
+
+# ---- right-hand sides ---------------------------------------------------------
+
+@pytest.mark.parametrize("N", [2, 3, 5, 8, 33, 130])
+def test_bruss_rhs_bitwise(rk, orc, N):
+    import torch
+    rng = np.random.default_rng(N)
+    p = rk.Bruss2D(N, 2.7e-3)
+    po = orc.make_ode("bruss2d", N, 2.7e-3)
+    y = rng.uniform(0.1, 5.0, p.n)
+    yd, fd = dev(y), torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    p.eval_rng_device(yd.data_ptr(), fd.data_ptr())
+    assert np.array_equal(fd.cpu().numpy(), orc.eval_rng(po, y))
+    # sub-range with absolute indexing (ode.h:35)
+    fd.zero_()
+    i, j = 3, p.n - 4
+    p.eval_rng_device(yd.data_ptr(), fd.data_ptr(), i, j)
+    got = fd.cpu().numpy()
+    ref = orc.eval_rng(po, y, i, j)
+    assert np.array_equal(got[i:j + 1], ref[i:j + 1]) and not got[:i].any()
+
+
+@pytest.mark.parametrize("kind", ["lv", "lvchain"])
+def test_lv_rhs_bitwise(rk, orc, kind):
+    import torch
+    rng = np.random.default_rng(5)
+    p = rk.LV(37) if kind == "lv" else rk.LVChain(37, diff=0.013)
+    po = orc.make_ode(kind, 37, diff=0.013)
+    y = rng.uniform(0.1, 2.0, p.n)
+    yd, fd = dev(y), torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    p.eval_rng_device(yd.data_ptr(), fd.data_ptr())
+    assert np.array_equal(fd.cpu().numpy(), orc.eval_rng(po, y))
+
+
+# ---- full solves, bitwise with host step control ------------------------------
+
+CFGS = [("D", 0, 0), ("PipeDls", 1, 16), ("PipeDls", 2, 7), ("PipeDls", 4, 32), ("PipeD", 4, 100)]
+
+
+@pytest.mark.parametrize("impl,waves,rows", CFGS)
+def test_c1_host_control_bitwise(rk, orc, impl, waves, rows):
+    """Config 1 (BRUSS2D N=128, DOPRI54, tol 1e-6, t in [0,4], h0 1e-3)."""
+    N = 128
+    po = orc.make_ode("bruss2d", N)
+    y0 = orc.ode_start(po)
+    yo, so, tro = orc.solve(po, "DOPRI54", 0.0, 4.0, y0, 1e-6, h0=1e-3, trace_cap=1000)
+    p = rk.Bruss2D(N)
+    y, st, tr = run_gpu(rk, p, "DOPRI54", 0.0, 4.0, y0, 1e-6, h0=1e-3, impl=impl, ctrl="host",
+                        tile_waves=waves, tile_rows=rows, trace_cap=1000)
+    assert (st.steps_acc, st.steps_rej) == (so["acc"], so["rej"]) == (320, 16)
+    assert np.array_equal(tr, tro)
+    assert np.array_equal(y, yo)
+
+
+@pytest.mark.parametrize("name", sorted(DUMPS))
+@pytest.mark.parametrize("impl", ["D", "PipeDls"])
+def test_golden_dump_host_control(rk, name, impl):
+    """Every attempted step's t and h equal the reference's DUMP output bitwise."""
+    meta = DUMPS[name]
+    d = meta["defines"]
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    prob = "bruss2d" if meta["problem"].startswith("bruss") else "lv"
+    grid = int(d.get("BRUSS_GRID_SIZE", d.get("LV_COUNT", 0)))
+    p = gpu_problem(rk, prob, grid)
+    m = rk.method(d["METHOD"])
+    tol = float(d["TOL"]) if m.has_emb else 0.0
+    y, st, tr = run_gpu(rk, p, m, 0.0, float(d["T_END"]), p.ode_start(), tol,
+                        h0=float(d["H_INIT"]), impl=impl, ctrl="host",
+                        trace_cap=len(z["t"]) + 5)
+    assert (st.steps_acc, st.steps_rej) == (meta["acc"], meta["rej"])
+    assert np.array_equal(tr[:, 0], z["t"]) and np.array_equal(tr[:, 1], z["h"])
+    assert [float(y[0]), float(y[-1]), float(y.min()), float(y.max())] == meta["footprint"]
+    assert np.array_equal(y, z["Y"][-1])
+
+
+# ---- device step control ----------------------------------------------------------
+
+KAT_CPU = [pytest.param(c, id=c["name"]) for c in KAT if not c.get("gpu_only")]
+
+
+@pytest.mark.parametrize("case", KAT_CPU)
+@pytest.mark.parametrize("impl", ["D", "PipeDls"])
+def test_kat_device_control(rk, orc, case, impl):
+    p = gpu_problem(rk, case["problem"], case["grid"], case.get("alpha", 2e-3))
+    y0 = p.ode_start()
+    y, st, tr = run_gpu(rk, p, case["method"], case["t0"], case["te"], y0, case["tol"],
+                        h0=case["h0"], step_limit=case["step_limit"], impl=impl, trace_cap=100000)
+    assert (st.steps_acc, st.steps_rej) == (case["acc"], case["rej"])
+    want = np.array([float(x) for x in case["footprint"]])
+    got = np.array([y[0], y[-1], y.min(), y.max()])
+    assert max_rel(got, want) <= REL_TOL
+    if case["grid"] <= 128:
+        po = orc.make_ode(case["problem"], case["grid"], case.get("alpha", 2e-3))
+        yo, so, tro = orc.solve(po, case["method"], case["t0"], case["te"], y0, case["tol"],
+                                h0=case["h0"], step_limit=case["step_limit"], trace_cap=100000)
+        assert max_rel(y, yo) <= REL_TOL
+        # the step sizes may differ by the device pow's last bit at most
+        assert max_rel(tr[:, 1], tro[:, 1]) <= 1e-13
+
+
+@pytest.mark.parametrize("case", [c for c in KAT if c.get("gpu_only")], ids=lambda c: c["name"])
+def test_c2_full_interval(rk, case):
+    """Config 2: BRUSS2D N=2000 over t in [0,1] (19.7 k / 15.7 k steps), against the
+    reference's full-run footprints and step counts."""
+    p = rk.Bruss2D(case["grid"], case["alpha"])
+    y, st, _ = run_gpu(rk, p, case["method"], case["t0"], case["te"], p.ode_start(), case["tol"],
+                       h0=case["h0"], impl="PipeDls")
+    assert (st.steps_acc, st.steps_rej) == (case["acc"], case["rej"])
+    want = np.array([float(x) for x in case["footprint"]])
+    got = np.array([y[0], y[-1], y.min(), y.max()])
+    assert max_rel(got, want) <= REL_TOL
+
+
+def test_zero_length_interval(rk):
+    p = rk.Bruss2D(8)
+    y0 = p.ode_start()
+    y, st, _ = run_gpu(rk, p, "DOPRI54", 1.0, 1.0, y0, 1e-6, h0=1e-3)
+    assert st.steps == 0 and np.array_equal(y, y0)
+
+
+def test_step_limit(rk):
+    p = rk.Bruss2D(40)
+    _, st, _ = run_gpu(rk, p, "DOPRI54", 0.0, 4.0, p.ode_start(), 1e-6, h0=1e-3, step_limit=37)
+    assert st.steps == 37
