@@ -94,9 +94,15 @@ enum {
 };
 
 enum {
-  RKX_CTRL_DEVICE = 0,   /* accept/reject and the new h on the device (fast) */
-  RKX_CTRL_HOST = 1      /* new h on the host with the C library's pow (bitwise
-                            reference step control; one host round trip per step) */
+  RKX_CTRL_DEVICE = 0,     /* step control on the device; the one pow() of the new h
+                              (include/tools.h:226) is answered by a host service
+                              thread with the C library's pow, so every h equals the
+                              reference's bitwise (default) */
+  RKX_CTRL_HOST = 1,       /* step control on the host after each step (one host
+                              round trip per step; bitwise as well) */
+  RKX_CTRL_DEVICE_POW = 2  /* step control and pow entirely on the device: fastest,
+                              but the device pow may differ from the C library's by
+                              an ulp, which changes the trajectory at round-off level */
 };
 
 typedef struct rkx_opts {

@@ -113,9 +113,36 @@ __device__ __forceinline__ void err_term(double e, double yn, double yo, double 
 }
 
 // ---- step control (include/tools.h:217-278, 191-198) ------------------------
-__device__ __forceinline__ double dev_h_new(double h, double err, double tol, int ord)
+// Iterations of the bounded wait for the host pow service (each one sleeps and
+// reads the mailbox over the host link, roughly a microsecond): about 0.2 s
+// in total, after which the step is handed to the host loop instead.
+constexpr int kPowSpin = 200000;
+
+// Posts pow(st->pow_x, st->pow_y) to the host service and waits, bounded, for
+// the answer.  Returns false on time-out.
+__device__ inline bool pow_service(state *st, double &p)
 {
-  return h * fmin(2.0, fmax(0.5, 0.9 * pow(tol / err, 1.0 / (ord + 1.0))));
+  pow_mailbox *mb = st->mb;
+  const unsigned long long seq = st->pow_seq + 1ull;
+  st->pow_seq = seq;
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(&mb->x),
+                     static_cast<unsigned long long>(__double_as_longlong(st->pow_x)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(&mb->y),
+                     static_cast<unsigned long long>(__double_as_longlong(st->pow_y)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mb->req, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  for (int it = 0; it < kPowSpin; ++it) {
+    if (__hip_atomic_load(&mb->resp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == seq) {
+      const unsigned long long pb = __hip_atomic_load(
+          reinterpret_cast<unsigned long long *>(&mb->p), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_SYSTEM);
+      p = __longlong_as_double(static_cast<long long>(pb));
+      return true;
+    }
+    __builtin_amdgcn_s_sleep(8);
+  }
+  return false;
 }
 
 // Runs in exactly one thread after every workgroup has folded its maximum into
@@ -125,45 +152,30 @@ __device__ inline void dev_control(state *st)
   const unsigned long long bits = atomicOr(&st->err_bits, 0ull);
   const unsigned int zero = atomicOr(&st->zero_div, 0u);
   const double q = __longlong_as_double(static_cast<long long>(bits));
-  const double err = zero ? fmax(1.7E308, q) : q;
   st->err_bits = 0ull;
   st->zero_div = 0u;
   st->blocks_done = 0u;
-  st->err = err;
-  if (st->host_ctrl)
+  st->err = zero ? fmax(1.7E308, q) : q;
+  if (st->ctrl == RKX_CTRL_HOST)
     return;
-  double t = st->t, h = st->h;
-  const double tol = st->tol, te = st->te;
   const int attempt = st->acc + st->rej;
   if (st->trace && attempt < st->trace_cap) {
-    st->trace[3 * attempt + 0] = t;
-    st->trace[3 * attempt + 1] = h;
-    st->trace[3 * attempt + 2] = err;
+    st->trace[3 * attempt + 0] = st->t;
+    st->trace[3 * attempt + 1] = st->h;
+    st->trace[3 * attempt + 2] = st->err;
   }
-  int accept;
-  if (tol <= 0.0) {
-    t += h;
-    accept = 1;
-  } else if (err <= tol) {
-    t += h;
-    h = dev_h_new(h, err, tol, st->ord);
-    accept = 1;
-  } else {
-    h = dev_h_new(h, err, tol, st->ord);
-    accept = 0;
+  const bool need = ctrl_decide(*st);
+  double p = 0.0;
+  if (need) {
+    if (st->ctrl == RKX_CTRL_DEVICE_POW) {
+      p = pow(st->pow_x, st->pow_y);
+    } else if (!pow_service(st, p)) {
+      st->pow_wait = 1;                // the host loop finishes this step's control
+      return;
+    }
   }
-  if (accept) {
-    st->acc += 1;
-    st->cur ^= 1;
-  } else {
-    st->rej += 1;
-  }
-  h = fmin(h, te - t);
-  st->t = t;
-  st->h = h;
-  premult(*st->meth, st->s, h, *st->coef);
-  const int attempts = st->acc + st->rej;
-  st->done = !((t < te) && (st->step_limit <= 0 || attempts < st->step_limit));
+  ctrl_finish(*st, need, p);
+  premult(*st->meth, st->s, st->h, *st->coef);
 }
 
 // ---- reductions -----------------------------------------------------------------

@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -46,35 +48,6 @@ dev_ode make_dev_ode(const rkx_ode &p)
   d.c1 = p.c1;
   d.diff = p.diff;
   return d;
-}
-
-// h_new_acc/h_new_rej and step_control (include/tools.h:221-278) with the C
-// library's pow, then the for-increment h = fmin(h, te - t) (tools.h:191-198).
-void host_control(state &st)
-{
-  const double err = st.err;
-  const int attempt = st.acc + st.rej;
-  (void) attempt;
-  int accept;
-  if (st.tol <= 0.0) {
-    st.t += st.h;
-    accept = 1;
-  } else if (err <= st.tol) {
-    st.t += st.h;
-    st.h = st.h * std::fmin(2.0, std::fmax(0.5, 0.9 * std::pow(st.tol / err, 1.0 / (st.ord + 1.0))));
-    accept = 1;
-  } else {
-    st.h = st.h * std::fmin(2.0, std::fmax(0.5, 0.9 * std::pow(st.tol / err, 1.0 / (st.ord + 1.0))));
-    accept = 0;
-  }
-  if (accept) {
-    st.acc += 1;
-    st.cur ^= 1;
-  } else {
-    st.rej += 1;
-  }
-  st.h = std::fmin(st.h, st.te - st.t);
-  st.done = !((st.t < st.te) && (st.step_limit <= 0 || st.acc + st.rej < st.step_limit));
 }
 
 namespace {
@@ -184,6 +157,79 @@ struct graph_holder {
       (void) hipGraphDestroy(g);
   }
 };
+
+// Host pow service: answers pow(x, y) requests that the device step control
+// posts in a mailbox of fine-grained pinned host memory, with the C library's
+// pow -- the one libm call of the reference's step control (tools.h:226), so
+// the device reproduces the reference's step sizes bitwise.  One busy-polling
+// thread per solve.
+class pow_service_thread {
+ public:
+  int start()
+  {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, sizeof(pow_mailbox), hipHostMallocCoherent | hipHostMallocMapped) !=
+        hipSuccess) {
+      (void) hipGetLastError();
+      return fail(RKX_ENOMEM, "hipHostMalloc of the pow mailbox failed");
+    }
+    mb_ = static_cast<pow_mailbox *>(p);
+    std::memset(mb_, 0, sizeof(*mb_));
+    void *dp = nullptr;
+    RKX_HIP(hipHostGetDevicePointer(&dp, p, 0));
+    dev_ = static_cast<pow_mailbox *>(dp);
+    stop_.store(false);
+    th_ = std::thread([this] { run(); });
+    return RKX_OK;
+  }
+  pow_mailbox *device_ptr() const { return dev_; }
+  ~pow_service_thread()
+  {
+    stop_.store(true);
+    if (th_.joinable())
+      th_.join();
+    if (mb_)
+      (void) hipHostFree(mb_);
+  }
+
+ private:
+  void run()
+  {
+    unsigned long long handled = 0;
+    while (!stop_.load(std::memory_order_relaxed)) {
+      const unsigned long long r = __atomic_load_n(&mb_->req, __ATOMIC_ACQUIRE);
+      if (r != handled) {
+        const double x = *reinterpret_cast<volatile double *>(&mb_->x);
+        const double y = *reinterpret_cast<volatile double *>(&mb_->y);
+        *reinterpret_cast<volatile double *>(&mb_->p) = std::pow(x, y);
+        __atomic_store_n(&mb_->resp, r, __ATOMIC_RELEASE);
+        handled = r;
+      } else {
+        __builtin_ia32_pause();
+      }
+    }
+  }
+  pow_mailbox *mb_ = nullptr;
+  pow_mailbox *dev_ = nullptr;
+  std::atomic<bool> stop_{false};
+  std::thread th_;
+};
+
+// Host half of the step control: ctrl_decide/ctrl_finish with the C library's
+// pow, then the premultiplied coefficients of the next step.
+void host_finish(state &st, bool decided, const dev_method &dm, dev_coef &hc)
+{
+  bool need;
+  if (!decided) {
+    need = ctrl_decide(st);
+  } else {
+    need = true;                       // decided on the device, waiting for pow
+    st.pow_wait = 0;
+  }
+  const double p = need ? std::pow(st.pow_x, st.pow_y) : 0.0;
+  ctrl_finish(st, need, p);
+  premult(dm, st.s, st.h, hc);
+}
 
 }  // namespace
 
@@ -371,6 +417,8 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     rkx_opts_default(&o);
   if (o.impl < RKX_IMPL_D || o.impl > RKX_IMPL_PIPEDLS)
     return fail(RKX_EINVAL, "rkx_solve: unknown impl");
+  if (o.ctrl < RKX_CTRL_DEVICE || o.ctrl > RKX_CTRL_DEVICE_POW)
+    return fail(RKX_EINVAL, "rkx_solve: unknown step-control mode");
   if (o.trace_cap < 0 || (o.trace_cap > 0 && !o.trace))
     return fail(RKX_EINVAL, "rkx_solve: trace_cap without a trace buffer");
   hipStream_t stream = static_cast<hipStream_t>(o.stream);
@@ -423,7 +471,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   hs.h = initial_h(te - t0, tol, o.step_limit, o.h0);
   hs.step_limit = o.step_limit;
   hs.ord = m->ord;
-  hs.host_ctrl = (o.ctrl == RKX_CTRL_HOST) ? 1 : 0;
+  hs.ctrl = o.ctrl;
   hs.trace = trace_dev;
   hs.trace_cap = o.trace_cap;
   hs.s = s;
@@ -433,6 +481,12 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   premult(dm, s, hs.h, hc);
   hs.cur = 0;
   hs.done = !(t0 < te);
+  pow_service_thread svc;
+  if (o.ctrl == RKX_CTRL_DEVICE && !hs.done) {
+    if (int rc = svc.start())
+      return rc;
+    hs.mb = svc.device_ptr();
+  }
 
   RKX_HIP(hipMemcpyAsync(dm_dev, &dm, sizeof(dm), hipMemcpyHostToDevice, stream));
   RKX_HIP(hipMemcpyAsync(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice, stream));
@@ -479,8 +533,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
         o.trace[3 * attempt + 1] = cur.h;
         o.trace[3 * attempt + 2] = cur.err;
       }
-      host_control(cur);
-      premult(dm, s, cur.h, hc);
+      host_finish(cur, false, dm, hc);
       RKX_HIP(hipMemcpyAsync(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice, stream));
       hs_pinned[0] = cur;
       RKX_HIP(hipMemcpyAsync(st_dev, hs_pinned, sizeof(state), hipMemcpyHostToDevice, stream));
@@ -514,17 +567,51 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
         (void) hipEventDestroy(e[1]);
       }
     } evf{ev};
-    RKX_HIP(hipGraphLaunch(gh.ge, stream));
-    RKX_HIP(hipMemcpyAsync(&hs_pinned[0], st_dev, sizeof(state), hipMemcpyDeviceToHost, stream));
-    RKX_HIP(hipEventRecord(ev[0], stream));
+    auto launch_chunk = [&](int slot) -> int {
+      RKX_HIP(hipGraphLaunch(gh.ge, stream));
+      RKX_HIP(hipMemcpyAsync(&hs_pinned[slot], st_dev, sizeof(state), hipMemcpyDeviceToHost,
+                             stream));
+      RKX_HIP(hipEventRecord(ev[slot], stream));
+      return RKX_OK;
+    };
+    if (int rc = launch_chunk(0))
+      return rc;
+    int last_attempts = -1, stalled = 0;
     for (int k = 0;; ++k) {
       const int a = k & 1, b = a ^ 1;
-      RKX_HIP(hipGraphLaunch(gh.ge, stream));
-      RKX_HIP(hipMemcpyAsync(&hs_pinned[b], st_dev, sizeof(state), hipMemcpyDeviceToHost, stream));
-      RKX_HIP(hipEventRecord(ev[b], stream));
+      if (int rc = launch_chunk(b))
+        return rc;
       RKX_HIP(hipEventSynchronize(ev[a]));
       if (hs_pinned[a].done)
         break;
+      if (hs_pinned[a].pow_wait) {
+        // The pow service did not answer in time: drain the stream (the
+        // remaining kernels return at once), finish this step's control on
+        // the host and restart the pipeline.
+        RKX_HIP(hipStreamSynchronize(stream));
+        state cur;
+        RKX_HIP(hipMemcpy(&cur, st_dev, sizeof(state), hipMemcpyDeviceToHost));
+        host_finish(cur, true, dm, hc);
+        RKX_HIP(hipMemcpy(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice));
+        RKX_HIP(hipMemcpy(st_dev, &cur, sizeof(state), hipMemcpyHostToDevice));
+        if (cur.done)
+          break;
+        if (int rc = launch_chunk(0))
+          return rc;
+        k = -1;
+        last_attempts = -1;
+        stalled = 0;
+        continue;
+      }
+      // every chunk must advance the attempt counter; a chunk that does not
+      // means the device-side step control is not running -- stop, do not spin
+      const int att = hs_pinned[a].acc + hs_pinned[a].rej;
+      stalled = (att == last_attempts) ? stalled + 1 : 0;
+      last_attempts = att;
+      if (stalled >= 3) {
+        (void) hipStreamSynchronize(stream);
+        return fail(RKX_EDEVICE, "device step control made no progress");
+      }
     }
     RKX_HIP(hipStreamSynchronize(stream));
     RKX_HIP(hipMemcpy(&hs_pinned[0], st_dev, sizeof(state), hipMemcpyDeviceToHost));

@@ -52,8 +52,19 @@ struct dev_coef {
   double he[RKX_MAX_STAGES];
 };
 
+// Mailbox of the host pow service, in fine-grained pinned host memory.  The
+// step control posts x, y and a new sequence number in req; the host service
+// thread answers p = pow(x, y) with the C library and echoes the sequence
+// number in resp.
+struct pow_mailbox {
+  unsigned long long req;
+  unsigned long long resp;
+  double x, y, p;
+  unsigned long long pad[3];
+};
+
 // Integration state of one solve, resident in device memory.  Written only by
-// the last workgroup of a step (or by the host in host-control mode).
+// the step control (the last workgroup of a step, or the host).
 struct alignas(64) state {
   double t, h, te, tol;
   double err;                        // error norm of the last attempt
@@ -65,9 +76,13 @@ struct alignas(64) state {
   int done;                          // the step loop has ended
   int step_limit;
   int ord;
-  int host_ctrl;                     // 1: last workgroup only finalizes err
+  int ctrl;                          // RKX_CTRL_*
+  int pow_wait;                      // 1: the new h waits for pow(pow_x, pow_y) from the host
   int trace_cap;
   int s;                             // stages
+  double pow_x, pow_y;               // pending pow arguments
+  unsigned long long pow_seq;        // last request number posted to the mailbox
+  pow_mailbox *mb;                   // host pow service mailbox (device-visible pointer)
   double *trace;                     // device buffer, 3 doubles per attempt
   const dev_method *meth;            // raw coefficients for the premultiplication
   dev_coef *coef;                    // premultiplied coefficients of the next step
@@ -84,9 +99,41 @@ __host__ __device__ inline void premult(const dev_method &m, int s, double h, de
   }
 }
 
-// Host-side step control, the bitwise reference form (include/tools.h:221-278
-// and the for-increment of FOR_ALL_GRIDPOINTS, tools.h:191-198).
-void host_control(state &st);
+// Step control of include/tools.h:235-278, split at the one libm call.
+// ctrl_decide: accept/reject with st.err, advance t and the counters, and set
+// pow_x = tol/err, pow_y = 1/(ord+1).  Returns true when the new h needs
+// p = pow(pow_x, pow_y) (variable step size).
+__host__ __device__ inline bool ctrl_decide(state &st)
+{
+  if (st.tol <= 0.0) {                 // fixed step size (tools.h:258-263)
+    st.t += st.h;
+    st.acc += 1;
+    st.cur ^= 1;
+    return false;
+  }
+  if (st.err <= st.tol) {              // accept (tools.h:264-270)
+    st.t += st.h;
+    st.acc += 1;
+    st.cur ^= 1;
+  } else {                             // reject: y stays in the other buffer
+    st.rej += 1;
+  }
+  st.pow_x = st.tol / st.err;
+  st.pow_y = 1.0 / (st.ord + 1.0);
+  return true;
+}
+
+// ctrl_finish: h_new_acc/h_new_rej with p (tools.h:221-233), then the
+// for-increment h = fmin(h, te - t) and the loop test of FOR_ALL_GRIDPOINTS
+// (tools.h:191-198).  The caller premultiplies the coefficients with the new h.
+__host__ __device__ inline void ctrl_finish(state &st, bool used_pow, double p)
+{
+  if (used_pow)
+    st.h = st.h * fmin(2.0, fmax(0.5, 0.9 * p));
+  st.h = fmin(st.h, st.te - st.t);
+  const int attempts = st.acc + st.rej;
+  st.done = !((st.t < st.te) && (st.step_limit <= 0 || attempts < st.step_limit));
+}
 
 // Kernel launchers (rkx_kernels_*.hip).  All return hipError_t of the launch.
 hipError_t launch_eval_rng(const dev_ode &p, int i, int j, const double *y, double *f,

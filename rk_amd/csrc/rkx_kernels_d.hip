@@ -28,7 +28,7 @@ __global__ __launch_bounds__(kBlock) void k_d_first(dev_ode p, const dev_coef *_
                                                     double *__restrict__ dy,
                                                     double *__restrict__ err)
 {
-  if (st->done)
+  if (st->done | st->pow_wait)
     return;
   const int cell = blockIdx.x * kBlock + threadIdx.x;
   if (cell >= p.cells)
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_d_interm(dev_ode p, const dev_coef *
                                                      double *__restrict__ dy,
                                                      double *__restrict__ err)
 {
-  if (st->done)
+  if (st->done | st->pow_wait)
     return;
   const int cell = blockIdx.x * kBlock + threadIdx.x;
   if (cell >= p.cells)
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_d_last(dev_ode p, const dev_coef *__
 {
   __shared__ double red[kBlock / 64];
   __shared__ unsigned int zred[kBlock / 64];
-  if (st->done)
+  if (st->done | st->pow_wait)
     return;                       // uniform: every block of the grid returns
   const int cell = blockIdx.x * kBlock + threadIdx.x;
   const int cur = st->cur;

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(dev_ode p,
   __shared__ double red[WAVES];
   __shared__ unsigned int zred[WAVES];
 
-  if (st->done)
+  if (st->done | st->pow_wait)
     return;                                   // uniform over the whole grid
   const int cur = st->cur;
   const double *__restrict__ y = cur ? y1 : y0;
