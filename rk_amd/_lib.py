@@ -3,7 +3,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_FILE = os.path.join(HERE, "librkx.so")
+# RKX_LIB selects another build of the same C ABI (e.g. an experimental kernel)
+_LIB_FILE = os.environ.get("RKX_LIB") or os.path.join(HERE, "librkx.so")
 
 RKX_MAX_STAGES = 16
 RKX_BRUSS2D, RKX_LV, RKX_LV_CHAIN = 1, 2, 3

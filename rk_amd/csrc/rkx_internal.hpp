@@ -50,6 +50,10 @@ struct dev_coef {
   double hA[RKX_MAX_STAGES * RKX_MAX_STAGES];
   double hb[RKX_MAX_STAGES];
   double he[RKX_MAX_STAGES];
+  // the same values grouped by the stage that consumes them: stage i scatters
+  // with col[i] = {hA[i+1][i], ..., hA[s-1][i], hb[i], he[i]} (contiguous, so
+  // a fused kernel loads one stage's coefficients with one scalar load)
+  double col[RKX_MAX_STAGES][RKX_MAX_STAGES + 2];
 };
 
 // Mailbox of the host pow service, in fine-grained pinned host memory.  The
@@ -96,6 +100,12 @@ __host__ __device__ inline void premult(const dev_method &m, int s, double h, de
       c.hA[i * RKX_MAX_STAGES + j] = h * m.A[i * RKX_MAX_STAGES + j];
     c.hb[i] = h * m.b[i];
     c.he[i] = h * m.e[i];
+  }
+  for (int i = 0; i < s; ++i) {
+    for (int l = i + 1; l < s; ++l)
+      c.col[i][l - i - 1] = c.hA[l * RKX_MAX_STAGES + i];
+    c.col[i][s - i - 1] = c.hb[i];
+    c.col[i][s - i] = c.he[i];
   }
 }
 

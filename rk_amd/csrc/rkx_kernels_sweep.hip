@@ -3,21 +3,25 @@
 // src/impl/seq/PipeDls.c:81-129) with block size B = one grid row, which is
 // the access distance d = 2N (bruss2d-mix.c:718-721).
 //
-// Work decomposition.  A workgroup of 64*W lanes owns a strip of grid columns
-// (one lane per grid point, u and v interleaved) and a band of R rows.  It
-// sweeps the band's rows in the block-diagonal order of PipeD: at row
-// iteration r, stage i is applied to row r - i, so when stage i needs w_i on
-// rows k-1, k, k+1, row k+1 has just been completed by stage i-1 in the same
-// iteration.  All per-row accumulators (w_{i+1..s-1}, dy, err of the scatter
-// form, block-inline.h:253-320) stay in registers in a ring of s+1 row slots;
-// the ring is fully unrolled so slots are static registers.
+// Work decomposition.  A workgroup owns a strip of L = 64*K grid columns and a
+// band of R rows.  It has 2K wavefronts: waves 0..K-1 carry the u component of
+// the strip's columns (one lane per column), waves K..2K-1 the v component of
+// the same columns, so each wave evaluates one of the two reaction formulas
+// without divergence and holds half of a grid point's accumulators.  The
+// workgroup sweeps the band's rows in the block-diagonal order of PipeD: at
+// row iteration r, stage i is applied to row r - i, so when stage i needs w_i
+// on rows k-1, k, k+1, row k+1 has just been completed by stage i-1 in the
+// same iteration.  The per-row accumulators of the scatter form
+// (w_{i+1..s-1}, dy, err; block-inline.h:253-320) stay in registers in a ring
+// of s+1 row slots; the ring is fully unrolled so slots are static registers.
 //
-// Neighbours.  Vertical neighbours are the lane's own registers; horizontal
-// ones are exchanged through LDS once per row iteration (double buffered, one
-// barrier).  Lanes at the strip edges and rows at the band edges are halo:
-// each stage's valid region shrinks by one grid point per stage, so a strip
-// produces 64*W - 2s output columns and a band recomputes s-1 rows on each
-// side (the stage-fused temporal blocking).
+// Neighbours.  Vertical neighbours are the lane's own registers.  Horizontal
+// neighbours and the other component of the same grid point are exchanged
+// through LDS once per row iteration (double buffered, one barrier).  Lanes
+// at the strip edges and rows at the band edges are halo: each stage's valid
+// region shrinks by one grid point per stage, so a strip produces L - 2s
+// output columns and a band recomputes s-1 rows on each side (the
+// stage-fused temporal blocking).
 //
 // Boundaries.  Rows above 0 / below N-1 are ghost rows loaded by reflection;
 // by symmetry they hold the mirrored values bitwise, which reproduces the
@@ -32,6 +36,9 @@
 // rejected step costs no copy.  Resident state is two vectors of n doubles.
 #include "rkx_device.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace rkx {
 
 namespace {
@@ -41,32 +48,51 @@ struct pat_dopri54 {
   static constexpr int s = 7;
   static constexpr unsigned rowA[7] = {0x0, 0x1, 0x3, 0x7, 0xF, 0x1F, 0x3D};
   static constexpr unsigned b = 0x3D, e = 0x7D;
+  static constexpr bool fsal = true;      // A[6][j] == b[j]
 };
 struct pat_verner65 {
   static constexpr int s = 8;
   static constexpr unsigned rowA[8] = {0x0, 0x1, 0x3, 0x7, 0xF, 0x1F, 0x1F, 0x5F};
   static constexpr unsigned b = 0x3D, e = 0xFD;
+  static constexpr bool fsal = false;
 };
 struct pat_rkf23 {
   static constexpr int s = 3;
   static constexpr unsigned rowA[3] = {0x0, 0x1, 0x3};
   static constexpr unsigned b = 0x3, e = 0x7;
+  static constexpr bool fsal = false;
 };
 struct pat_heun2 {
   static constexpr int s = 2;
   static constexpr unsigned rowA[2] = {0x0, 0x1};
   static constexpr unsigned b = 0x3, e = 0x3;
+  static constexpr bool fsal = false;
 };
 struct pat_rk4 {
   static constexpr int s = 4;
   static constexpr unsigned rowA[4] = {0x0, 0x1, 0x2, 0x4};
   static constexpr unsigned b = 0xF, e = 0xF;
+  static constexpr bool fsal = false;
 };
 struct pat_ssprk3 {
   static constexpr int s = 3;
   static constexpr unsigned rowA[3] = {0x0, 0x1, 0x3};
   static constexpr unsigned b = 0x7, e = 0x7;
+  static constexpr bool fsal = false;
 };
+
+// compile-time loop: f(integral_constant<int, I>) for I = 0 .. N-1, so ring
+// slots and stage indices are constants and the register ring stays in VGPRs
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Is...>)
+{
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <class P>
 __device__ __forceinline__ constexpr bool nzA(int l, int i)
@@ -74,46 +100,33 @@ __device__ __forceinline__ constexpr bool nzA(int l, int i)
   return (P::rowA[l] >> i) & 1u;
 }
 
+// even periodic extension of a row/column index about 0 and N-1 (ghost
+// rows and columns lie at most a few tiles' halo beyond the edge; the loop
+// covers grids narrower than the halo as well)
 __device__ __forceinline__ int reflect(int k, int N)
 {
-  // even periodic extension about rows 0 and N-1
-  if (N == 1)
-    return 0;
-  const int period = 2 * (N - 1);
-  int m = k % period;
-  if (m < 0)
-    m += period;
-  return (m < N) ? m : period - m;
+  while (k < 0 || k > N - 1)
+    k = (k < 0) ? -k : 2 * (N - 1) - k;
+  return k;
 }
 
-__device__ __forceinline__ double2 ld2(const double *p)
+template <class P, int K>
+__global__ __launch_bounds__(128 * K) void k_sweep(dev_ode p, const dev_coef *__restrict__ cf,
+                                                   state *st, double *y0, double *y1, int R,
+                                                   int OC, int nstrips, int nbands)
 {
-  return *reinterpret_cast<const double2 *>(p);
-}
-
-template <class P, int WAVES>
-struct sweep {
-  static constexpr int s = P::s;
-  static constexpr int L = 64 * WAVES;     // lanes = columns per strip
-  static constexpr int NS = s + 1;         // row slots in the register ring
-  static constexpr int LP = L + 3;         // + AUX_L, AUX_R, ZERO slots
-  static constexpr int AUXL = L, AUXR = L + 1, ZERO = L + 2;
-};
-
-template <class P, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_sweep(dev_ode p,
-                                                      const dev_coef *__restrict__ cf,
-                                                      state *st, double *y0, double *y1,
-                                                      int R, int OC, int nstrips, int nbands)
-{
-  using SW = sweep<P, WAVES>;
-  constexpr int s = SW::s, L = SW::L, NS = SW::NS, LP = SW::LP;
-  __shared__ double2 xch[2][s][LP];
-  __shared__ double red[WAVES];
-  __shared__ unsigned int zred[WAVES];
+  constexpr int s = P::s;
+  constexpr int L = 64 * K;          // columns per strip
+  constexpr int NS = s + 1;          // row slots in the register ring
+  constexpr int LP = L + 1;          // + a zero slot
+  constexpr int ZERO = L;
+  constexpr int PF = (s - 2 < 2) ? s - 2 : 2;   // y rows loaded ahead of stage 0
+  __shared__ double xch[2][s][2][LP];
+  __shared__ double red[2 * K];
+  __shared__ unsigned int zred[2 * K];
 
   if (st->done | st->pow_wait)
-    return;                                   // uniform over the whole grid
+    return;                                      // uniform over the whole grid
   const int cur = st->cur;
   const double *__restrict__ y = cur ? y1 : y0;
   double *__restrict__ yn = cur ? y0 : y1;
@@ -131,168 +144,194 @@ __global__ __launch_bounds__(64 * WAVES) void k_sweep(dev_ode p,
   const int strip = tile / nbands;
   const int band = tile - strip * nbands;
 
-  const int lane = threadIdx.x;
-  const int col = strip * OC - s + lane;
-  const int ccol = reflect(col, N);          // in-bounds column for loads
-  const bool out_col = (lane >= s) && (lane < L - s) && (col >= 0) && (col < N) &&
+  const int var = (threadIdx.x >= L) ? 1 : 0;   // wave-uniform: 0 = u, 1 = v
+  const int cl = threadIdx.x - var * L;         // column lane within the strip
+  const int col = strip * OC - s + cl;
+  const int ccol = reflect(col, N);             // in-bounds column for loads
+  const bool out_col = (cl >= s) && (cl < L - s) && (col >= 0) && (col < N) &&
                        (col < (strip + 1) * OC);
-  // LDS read slots for the W and E neighbours (see the file comment)
-  const int slotW = (col == 0) ? SW::ZERO : ((col == N - 1) ? SW::AUXR : max(lane - 1, 0));
-  const int slotE = (col == 0) ? SW::AUXL : ((col == N - 1) ? SW::ZERO : min(lane + 1, L - 1));
-  const bool wr_auxl = (col == 1);           // 2*w of column 1 feeds column 0
-  const bool wr_auxr = (col == N - 2);       // 2*w of column N-2 feeds column N-1
-  if (lane == 0) {
-    for (int bb = 0; bb < 2; ++bb)
-      for (int i = 0; i < s; ++i)
-        xch[bb][i][SW::ZERO] = make_double2(0.0, 0.0);
+  // Horizontal part of the Laplacian: ((vN + vS) + X) + m*Y.  Interior
+  // columns read X = W, Y = E, m = 1.  Column 0 reads X from the zero slot and
+  // Y = E with m = 2; column N-1 reads X from the zero slot and Y = W with
+  // m = 2.  Scaling by 1 or 2 is exact, so every lane computes the
+  // reference's expression for its column (bruss2d-mix.c:66-206).
+  const int slotX = (col == 0 || col == N - 1) ? ZERO : max(cl - 1, 0);
+  const int slotY = (col == N - 1) ? max(cl - 1, 0) : min(cl + 1, L - 1);
+  const double mY = (col == 0 || col == N - 1) ? 2.0 : 1.0;
+  if (threadIdx.x < 2 * s) {
+    const int bb = threadIdx.x / s, i = threadIdx.x - bb * s;
+    xch[bb][i][0][ZERO] = 0.0;
+    xch[bb][i][1][ZERO] = 0.0;
   }
 
   const int r0 = band * R;
   const int r1 = min(r0 + R, N);
-  const int it0 = r0 - (s - 1);               // first row iteration
-  const int it1 = r1 + s - 1;                 // one past the last
+  const int it0 = r0 - (s - 1);                  // first row iteration
+  const int it1 = r1 + s - 1;                    // one past the last
 
-  // uniform premultiplied coefficients
-  double hA[s][s];
-  double hb[s], he[s];
-#pragma unroll
-  for (int l = 0; l < s; ++l) {
-#pragma unroll
-    for (int i = 0; i < s; ++i)
-      hA[l][i] = (i < l && nzA<P>(l, i)) ? cf->hA[l * RKX_MAX_STAGES + i] : 0.0;
-    hb[l] = ((P::b >> l) & 1u) ? cf->hb[l] : 0.0;
-    he[l] = ((P::e >> l) & 1u) ? cf->he[l] : 0.0;
-  }
+  // register ring: slot -> [0] dy, [1..s-1] w_l, [s] err (this lane's component)
+  double acc[NS][s + 1];
+  double yr[NS];                                 // y rows of stage 0, ring by row
+  double yo[NS];                                 // y_old of finishing rows, ring by iteration
+  constexpr int DY = 2;                          // y_old loaded DY iterations ahead
+  const double *yc = y + 2 * ccol + var;
+  auto yrow = [&](int k) -> double { return yc[static_cast<size_t>(reflect(k, N)) * N2]; };
+  static_for<PF + 2>([&](auto dt) {
+    constexpr int d = decltype(dt)::value - 1;   // rows it0-1 .. it0+PF
+    yr[(d + NS) % NS] = yrow(it0 + d);
+  });
+  static_for<NS>([&](auto dt) {
+    constexpr int d = decltype(dt)::value;
+    yo[d] = 0.0;
+    static_for<s + 1>([&](auto et) { acc[d][decltype(et)::value] = 0.0; });
+  });
 
-  // register ring: slot -> [0] dy, [1..s-1] w_l, [s] err
-  double2 acc[NS][s + 1];
-  double2 yr[NS];                            // y rows, ring by row
-  constexpr int PF = (s - 2 < 2) ? s - 2 : 2; // y rows loaded ahead of stage 0 (< NS-2)
-
-  auto yrow = [&](int k) -> const double * {
-    return y + static_cast<size_t>(reflect(k, N)) * N2 + 2 * ccol;
-  };
-  // rows it0-1 .. it0+PF are loaded before the loop
-#pragma unroll
-  for (int d = -1; d <= PF; ++d)
-    yr[(d + NS) % NS] = ld2(yrow(it0 + d));
+  // The step's premultiplied coefficients, loaded once (uniform, SGPRs):
+  // kv[i] = col[i] = {hA[i+1..s-1][i], hb[i], he[i]} for the scatter of stage
+  // i.  Only the non-zero entries are loaded; for a first-same-as-last table
+  // (A[s-1][j] == b[j], e.g. DOPRI54) h*A[s-1][j] and h*b[j] are the same
+  // product, so hb[j] reuses the register of hA[s-1][j].
+  double kv[s][s + 1];
+  static_for<s>([&](auto it) {
+    constexpr int i = decltype(it)::value;
+    static_for<s>([&](auto lt) {
+      constexpr int l = decltype(lt)::value;
+      if constexpr (l > i)
+        kv[i][l - i - 1] = nzA<P>(l, i) ? cf->col[i][l - i - 1] : 0.0;
+    });
+    if constexpr (P::fsal && i < s - 1)
+      kv[i][s - i - 1] = kv[i][s - i - 2];
+    else
+      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? cf->col[i][s - i - 1] : 0.0;
+    kv[i][s - i] = ((P::e >> i) & 1u) ? cf->col[i][s - i] : 0.0;
+  });
 
   double qmax = 0.0;
   unsigned int zero = 0u;
 
-  for (int rb = it0; rb < it1; rb += NS) {
-#pragma unroll
-    for (int U = 0; U < NS; ++U) {
-      const int rr = rb + U;
-      if (rr >= it1)
-        break;
-      // prefetch y for stage 0 of a later row
-      yr[(U + 1 + PF) % NS] = ld2(yrow(rr + 1 + PF));
-      // y of the row finishing this iteration (stage s-1), for y_new and the norm
-      const int kl = rr - (s - 1);
-      const bool last_on = (rr >= it0 + 2 * (s - 1));
-      double2 yold = make_double2(0.0, 0.0);
-      if (last_on)
-        yold = ld2(y + static_cast<size_t>(kl) * N2 + 2 * ccol);
+  // The row loop, instantiated once for the u waves and once for the v
+  // waves (VAR), so that each is straight-line code per row iteration.
+  auto rows = [&](auto vt) {
+    constexpr int VAR = decltype(vt)::value;
+    for (int rb = it0; rb < it1; rb += NS) {
+      static_for<NS>([&](auto ut) {
+        constexpr int U = decltype(ut)::value;
+        const int rr = rb + U;
+        if (rr >= it1)
+          return;
+        // Loads complete in issue order, so every load used in this
+        // iteration was issued in an earlier one: y for stage 0 PF rows
+        // ahead, and y_old of the row the last stage finishes DY iterations
+        // ahead (clamped into the band: rows outside it are never stored).
+        const int kf = min(max(rr + DY - (s - 1), r0), r1 - 1);
+        yo[(U + DY) % NS] = yc[static_cast<size_t>(kf) * N2];
+        yr[(U + 1 + PF) % NS] = yrow(rr + 1 + PF);
+        const int kl = rr - (s - 1);                 // row finishing this iteration
+        const double yold = yo[U % NS];
 
-      // publish the centre value of every active stage
-      const int buf = (rr - it0) & 1;
-#pragma unroll
-      for (int i = 0; i < s; ++i) {
-        if (rr >= it0 + 2 * i) {
-          const double2 c = (i == 0) ? yr[U % NS] : acc[(U - i + NS) % NS][i];
-          xch[buf][i][lane] = c;
-          if (wr_auxl)
-            xch[buf][i][SW::AUXL] = make_double2(c.x + c.x, c.y + c.y);
-          if (wr_auxr)
-            xch[buf][i][SW::AUXR] = make_double2(c.x + c.x, c.y + c.y);
-        }
-      }
-      __syncthreads();
+        // Every stage runs every iteration: during the first 2(s-1)
+        // iterations of a band, stage i works on rows above its range, whose
+        // results are never read by an in-range row and never stored.
+        const int buf = (rr - it0) & 1;
+        static_for<s>([&](auto itg) {
+          constexpr int i = decltype(itg)::value;
+          xch[buf][i][VAR][cl] = (i == 0) ? yr[U % NS] : acc[(U - i + NS) % NS][i];
+        });
+        __syncthreads();
+        double nX[s], nY[s], nP[s];
+        static_for<s>([&](auto itg) {
+          constexpr int i = decltype(itg)::value;
+          nX[i] = xch[buf][i][VAR][slotX];
+          nY[i] = xch[buf][i][VAR][slotY];
+          nP[i] = xch[buf][i][VAR ^ 1][cl];           // the other component
+        });
 
-#pragma unroll
-      for (int i = 0; i < s; ++i) {
-        if (!(rr >= it0 + 2 * i))
-          continue;
-        const int sl = (U - i + NS) % NS;         // slot of row rr - i
-        const int sn = (U - i - 1 + 2 * NS) % NS; // row rr - i - 1
-        const int ss = (U - i + 1 + NS) % NS;     // row rr - i + 1
-        double2 c, vn, vs;
-        if (i == 0) {
-          c = yr[U % NS];
-          vn = yr[(U - 1 + NS) % NS];
-          vs = yr[(U + 1) % NS];
-        } else {
-          c = acc[sl][i];
-          vn = acc[sn][i];
-          vs = acc[ss][i];
-        }
-        const double2 vw = xch[buf][i][slotW];
-        const double2 ve = xch[buf][i][slotE];
-        // Laplacian numerators: ((vN + vS) + vW) + vE - 4c
-        const double lu = __builtin_fma(-4.0, c.x, ((vn.x + vs.x) + vw.x) + ve.x);
-        const double lv = __builtin_fma(-4.0, c.y, ((vn.y + vs.y) + vw.y) + ve.y);
-        double fu, fv;
-        bruss_react(c.x, c.y, lu, lv, coef, fu, fv);
-
-        if (i == 0) {
-          // first stage: w_l = y + hA[l][0] F, dy = hb0 F, err = he0 F
-#pragma unroll
-          for (int l = 1; l < s; ++l) {
-            if (nzA<P>(l, 0))
-              acc[sl][l] = make_double2(c.x + hA[l][0] * fu, c.y + hA[l][0] * fv);
-            else
-              acc[sl][l] = c;
+        static_for<s>([&](auto itg) {
+          constexpr int i = decltype(itg)::value;
+          constexpr int sl = (U - i + NS) % NS;              // slot of row rr - i
+          constexpr int sn = (U - i - 1 + 2 * NS) % NS;      // row rr - i - 1
+          constexpr int ss = (U - i + 1 + NS) % NS;          // row rr - i + 1
+          double c, vn, vs;
+          if constexpr (i == 0) {
+            c = yr[U % NS];
+            vn = yr[(U - 1 + NS) % NS];
+            vs = yr[(U + 1) % NS];
+          } else {
+            c = acc[sl][i];
+            vn = acc[sn][i];
+            vs = acc[ss][i];
           }
-          acc[sl][0] = make_double2(hb[0] * fu, hb[0] * fv);
-          acc[sl][s] = make_double2(he[0] * fu, he[0] * fv);
-        } else {
-#pragma unroll
-          for (int l = i + 1; l < s; ++l)
-            if (nzA<P>(l, i)) {
-              acc[sl][l].x += hA[l][i] * fu;
-              acc[sl][l].y += hA[l][i] * fv;
-            }
-          if ((P::b >> i) & 1u) {
-            acc[sl][0].x += hb[i] * fu;
-            acc[sl][0].y += hb[i] * fv;
+          const double pt = nP[i];
+          // Laplacian numerator ((vN + vS) + X) + m*Y - 4c, then the reaction
+          const double lap = __builtin_fma(-4.0, c, ((vn + vs) + nX[i]) + mY * nY[i]);
+          double f;
+          if constexpr (VAR == 0) {
+            const double uuv = (c * c) * pt;
+            f = ((1.0 + uuv) - 4.4 * c) + coef * lap;
+          } else {
+            const double uuv = (pt * pt) * c;
+            f = (3.4 * pt - uuv) + coef * lap;
           }
-          if ((P::e >> i) & 1u) {
-            acc[sl][s].x += he[i] * fu;
-            acc[sl][s].y += he[i] * fv;
+          const double *kc = kv[i];
+          if constexpr (i == 0) {
+            // first stage: w_l = y + hA[l][0] F, dy = hb0 F, err = he0 F
+            static_for<s>([&](auto ltg) {
+              constexpr int l = decltype(ltg)::value;
+              if constexpr (l >= 1)
+                acc[sl][l] = nzA<P>(l, 0) ? c + kc[l - 1] * f : c;
+            });
+            acc[sl][0] = kc[s - 1] * f;
+            acc[sl][s] = kc[s] * f;
+          } else {
+            static_for<s>([&](auto ltg) {
+              constexpr int l = decltype(ltg)::value;
+              if constexpr (l > i && nzA<P>(l, i))
+                acc[sl][l] += kc[l - i - 1] * f;
+            });
+            if constexpr ((P::b >> i) & 1u)
+              acc[sl][0] += kc[s - i - 1] * f;
+            if constexpr ((P::e >> i) & 1u)
+              acc[sl][s] += kc[s - i] * f;
           }
-        }
-        if (i == s - 1) {
-          // y_new = y + dy; dy now holds y_old (block-inline.h:298-320)
-          double2 ynw;
-          ynw.x = yold.x + acc[sl][0].x;
-          ynw.y = yold.y + acc[sl][0].y;
-          if (out_col) {
-            *reinterpret_cast<double2 *>(yn + static_cast<size_t>(kl) * N2 + 2 * col) = ynw;
-            err_term(acc[sl][s].x, ynw.x, yold.x, qmax, zero);
-            err_term(acc[sl][s].y, ynw.y, yold.y, qmax, zero);
+          if constexpr (i == s - 1) {
+            // y_new = y + dy; the error term against y_old (block-inline.h:298-320)
+            const double ynw = yold + acc[sl][0];
+            const bool out = out_col && (kl >= r0);
+            if (out)
+              yn[static_cast<size_t>(kl) * N2 + 2 * col + VAR] = ynw;
+            // without branches: q = |e| / max(|y_new|, |y_old|); a zero
+            // divisor raises the sentinel, NaN quotients are skipped by fmax
+            const double dv = fmax(fabs(ynw), fabs(yold));
+            const double qv = fabs(acc[sl][s]) / dv;
+            const bool dz = (dv == 0.0);
+            zero |= (out && dz) ? 1u : 0u;
+            qmax = (out && !dz) ? fmax(qmax, qv) : qmax;
           }
-        }
-      }
+        });
+      });
     }
-  }
+  };
+  if (__builtin_amdgcn_readfirstlane(var) == 0)
+    rows(std::integral_constant<int, 0>{});
+  else
+    rows(std::integral_constant<int, 1>{});
   block_finish(st, qmax, zero, red, zred);
 }
 
-template <class P, int WAVES>
+template <class P, int K>
 hipError_t launch_one(const dev_ode &p, const dev_coef *cf, state *st, double *y0, double *y1,
                       int R, hipStream_t stream)
 {
   constexpr int s = P::s;
-  constexpr int L = 64 * WAVES;
+  constexpr int L = 64 * K;
   const int N = p.N;
   const int oc_max = L - 2 * s;
   const int nstrips = (N + oc_max - 1) / oc_max;
   const int OC = (N + nstrips - 1) / nstrips;
   const int nbands = (N + R - 1) / R;
   const int grid = nstrips * nbands;
-  hipLaunchKernelGGL((k_sweep<P, WAVES>), dim3(grid), dim3(L), 0, stream, p, cf, st, y0, y1,
-                     R, OC, nstrips, nbands);
+  hipLaunchKernelGGL((k_sweep<P, K>), dim3(grid), dim3(2 * L), 0, stream, p, cf, st, y0, y1, R,
+                     OC, nstrips, nbands);
   return hipGetLastError();
 }
 
@@ -321,7 +360,10 @@ int pattern_of(const rkx_method &m)
         return false;
     return vmask(m.b, nullptr) == b && vmask(m.b, m.b_hat) == e;
   };
-  if (match(7, pat_dopri54::rowA, pat_dopri54::b, pat_dopri54::e)) return 1;
+  bool fsal = true;
+  for (int j = 0; j < m.s; ++j)
+    fsal = fsal && (m.A[m.s - 1][j] == m.b[j] || j == m.s - 1);
+  if (match(7, pat_dopri54::rowA, pat_dopri54::b, pat_dopri54::e) && fsal) return 1;
   if (match(8, pat_verner65::rowA, pat_verner65::b, pat_verner65::e)) return 2;
   if (match(3, pat_rkf23::rowA, pat_rkf23::b, pat_rkf23::e)) return 3;
   if (match(2, pat_heun2::rowA, pat_heun2::b, pat_heun2::e)) return 4;
@@ -337,6 +379,7 @@ hipError_t launch_pat(const dev_ode &p, const dev_coef *cf, state *st, double *y
   switch (c.waves) {
   case 1: return launch_one<P, 1>(p, cf, st, y0, y1, c.rows, stream);
   case 2: return launch_one<P, 2>(p, cf, st, y0, y1, c.rows, stream);
+  case 3: return launch_one<P, 3>(p, cf, st, y0, y1, c.rows, stream);
   default: return launch_one<P, 4>(p, cf, st, y0, y1, c.rows, stream);
   }
 }
@@ -348,16 +391,24 @@ bool sweep_supported(const dev_ode &p, const rkx_method &m)
   return p.kind == RKX_BRUSS2D && p.N >= 2 && pattern_of(m) != 0;
 }
 
+// Default strip width K (64*K columns, 2K waves) and band height R: about two
+// resident workgroups per CU over the whole grid, one dispatch round.
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
 {
-  (void) m;
   sweep_cfg c;
-  c.waves = 4;
-  c.rows = 32;
-  if (p.N <= 256) {
-    c.waves = 2;
-    c.rows = 16;
-  }
+  const int s = m.s;
+  const int N = p.N;
+  c.waves = (N <= 128) ? 1 : ((N <= 512) ? 2 : 3);
+  const int L = 64 * c.waves;
+  const int oc = L - 2 * s;
+  const int nstrips = (oc > 0) ? (N + oc - 1) / oc : 1;
+  const int target_wg = 512;                           // ~2 per CU on 256 CUs
+  int bands = target_wg / (nstrips > 0 ? nstrips : 1);
+  if (bands < 1)
+    bands = 1;
+  c.rows = (N + bands - 1) / bands;
+  if (c.rows < 8)
+    c.rows = 8;
   return c;
 }
 

@@ -70,10 +70,12 @@ def test_bruss_rhs_bitwise(rk, orc, N):
     po = orc.make_ode("bruss2d", N, 2.7e-3)
     y = rng.uniform(0.1, 5.0, p.n)
     yd, fd = dev(y), torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
     p.eval_rng_device(yd.data_ptr(), fd.data_ptr())
     assert np.array_equal(fd.cpu().numpy(), orc.eval_rng(po, y))
     # sub-range with absolute indexing (ode.h:35)
     fd.zero_()
+    torch.cuda.synchronize()          # the library runs on its own stream
     i, j = 3, p.n - 4
     p.eval_rng_device(yd.data_ptr(), fd.data_ptr(), i, j)
     got = fd.cpu().numpy()
@@ -89,6 +91,7 @@ def test_lv_rhs_bitwise(rk, orc, kind):
     po = orc.make_ode(kind, 37, diff=0.013)
     y = rng.uniform(0.1, 2.0, p.n)
     yd, fd = dev(y), torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
     p.eval_rng_device(yd.data_ptr(), fd.data_ptr())
     assert np.array_equal(fd.cpu().numpy(), orc.eval_rng(po, y))
 
