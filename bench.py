@@ -81,6 +81,11 @@ def main():
     ap.add_argument("--tile-waves", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--step-limit", type=int, default=0,
+                    help="cap each solve at this many attempted steps (profiling runs)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run 5 plain device copies of n doubles, the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -98,7 +103,11 @@ def main():
     y = torch.empty_like(y0)
     torch.cuda.synchronize()
     kw = dict(h0=1e-3, impl=a.impl, tile_waves=a.tile_waves, tile_rows=a.tile_rows,
-              stream=stream.cuda_stream)
+              stream=stream.cuda_stream, step_limit=a.step_limit)
+    if a.calib_copy:
+        for _ in range(5):
+            y.copy_(y0)                  # reads 8n bytes, writes 8n bytes
+        torch.cuda.synchronize()
 
     for _ in range(a.warmup):
         st, _ = rk_amd.solve(p, a.method, 0.0, 1.0, y0, y, a.tol, **kw)

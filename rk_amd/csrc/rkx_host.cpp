@@ -493,7 +493,9 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   RKX_HIP(hipMemcpyAsync(st_dev, &hs, sizeof(hs), hipMemcpyHostToDevice, stream));
   RKX_HIP(hipMemcpyAsync(ybuf[0], y0_dev, vec, hipMemcpyDeviceToDevice, stream));
 
-  sweep_cfg cfg = sweep_default_cfg(dp, *m);
+  sweep_cfg cfg{1, 1};
+  if (impl != RKX_IMPL_D)
+    cfg = sweep_default_cfg(dp, *m);
   if (o.tile_rows > 0)
     cfg.rows = o.tile_rows;
   if (o.tile_waves > 0)

@@ -384,6 +384,40 @@ hipError_t launch_pat(const dev_ode &p, const dev_coef *cf, state *st, double *y
   }
 }
 
+template <class P, int K>
+int blocks_per_cu()
+{
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<P, K>, 128 * K, 0) != hipSuccess) {
+    (void) hipGetLastError();
+    nb = 1;
+  }
+  return nb > 0 ? nb : 1;
+}
+
+template <class P>
+int blocks_per_cu_k(int K)
+{
+  switch (K) {
+  case 1: return blocks_per_cu<P, 1>();
+  case 2: return blocks_per_cu<P, 2>();
+  case 3: return blocks_per_cu<P, 3>();
+  default: return blocks_per_cu<P, 4>();
+  }
+}
+
+int resident_blocks(int pattern, int K)
+{
+  switch (pattern) {
+  case 1: return blocks_per_cu_k<pat_dopri54>(K);
+  case 2: return blocks_per_cu_k<pat_verner65>(K);
+  case 3: return blocks_per_cu_k<pat_rkf23>(K);
+  case 4: return blocks_per_cu_k<pat_heun2>(K);
+  case 5: return blocks_per_cu_k<pat_rk4>(K);
+  default: return blocks_per_cu_k<pat_ssprk3>(K);
+  }
+}
+
 }  // namespace
 
 bool sweep_supported(const dev_ode &p, const rkx_method &m)
@@ -391,24 +425,34 @@ bool sweep_supported(const dev_ode &p, const rkx_method &m)
   return p.kind == RKX_BRUSS2D && p.N >= 2 && pattern_of(m) != 0;
 }
 
-// Default strip width K (64*K columns, 2K waves) and band height R: about two
-// resident workgroups per CU over the whole grid, one dispatch round.
+// Default tile: strips of 64*K columns (2K waves per workgroup) and the band
+// height R for which all strips x bands tiles are resident at once (one
+// dispatch round: a second, partial round would leave most CUs idle while
+// it runs), with bands as tall as that allows (the s-1 halo rows a band
+// recomputes are a smaller share of taller bands).
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
 {
   sweep_cfg c;
   const int s = m.s;
   const int N = p.N;
-  c.waves = (N <= 128) ? 1 : ((N <= 512) ? 2 : 3);
+  c.waves = (N <= 64) ? 1 : 2;
   const int L = 64 * c.waves;
   const int oc = L - 2 * s;
-  const int nstrips = (oc > 0) ? (N + oc - 1) / oc : 1;
-  const int target_wg = 512;                           // ~2 per CU on 256 CUs
-  int bands = target_wg / (nstrips > 0 ? nstrips : 1);
+  const int nstrips = (N > 0) ? (N + oc - 1) / oc : 1;
+  int cus = 0, dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1) {
+    (void) hipGetLastError();
+    cus = 256;
+  }
+  const int slots = cus * resident_blocks(pattern_of(m), c.waves);
+  int bands = slots / nstrips;
   if (bands < 1)
     bands = 1;
   c.rows = (N + bands - 1) / bands;
-  if (c.rows < 8)
-    c.rows = 8;
+  if (c.rows < 4)
+    c.rows = 4;
   return c;
 }
 
