@@ -139,6 +139,58 @@ int rkx_solve_host(const rkx_ode *p, const rkx_method *m, double t0, double te,
                    const double *y0, double *y, double tol, const rkx_opts *o,
                    rkx_stats *st);
 
+/* ---- batched solves (no reference counterpart: the reference integrates one
+ * system per process).  Many instances of uneven size share each launch: one
+ * batched step advances every active instance by one attempted step with its
+ * own t, h, accept/reject and step limit, so each instance's result is the
+ * same bitwise whether it runs alone or in a mix.  Instances can be added and
+ * removed between steps (continuous batching). ------------------------------ */
+typedef struct rkx_batch rkx_batch;
+
+enum {
+  RKX_STORAGE_AUTO = 0,   /* low storage when a fused kernel exists, else full */
+  RKX_STORAGE_LOW = 1,    /* fused kernel, two vectors of n (PipeDls form) */
+  RKX_STORAGE_FULL = 2    /* stage kernels, (s+3) vectors of n (D form) */
+};
+
+typedef struct rkx_batch_opts {
+  size_t hbm_budget;      /* device bytes the batch may hold for instance data (0: no limit) */
+  int ctrl;               /* RKX_CTRL_DEVICE or RKX_CTRL_DEVICE_POW */
+  int tile_waves;         /* fused-kernel strip width for all instances (0: chosen per size) */
+  int tile_rows;          /* fused-kernel band height (0: chosen per size) */
+  void *stream;           /* hipStream_t, NULL: the library's own stream */
+} rkx_batch_opts;
+
+void rkx_batch_opts_default(rkx_batch_opts *o);
+int rkx_batch_create(rkx_batch **b, const rkx_batch_opts *o);
+int rkx_batch_destroy(rkx_batch *b);
+
+/* Device bytes an instance would hold in the given storage mode. */
+size_t rkx_batch_footprint(const rkx_ode *p, const rkx_method *m, int storage);
+
+/* Add an instance starting from host y0 at t0.  Returns its id (>= 0), or
+ * RKX_ELIMIT when it does not fit the HBM budget now (retry after removing
+ * finished instances), or another negative code. */
+int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0, double te,
+                  const double *y0, double tol, double h0, int step_limit, int storage);
+
+/* Run up to max_steps batched steps (stops early when every instance is done).
+ * *active receives the number of instances still integrating. */
+int rkx_batch_run(rkx_batch *b, int max_steps, int *active);
+
+/* Per-instance statistics; *done = 1 once the instance reached te or its
+ * step limit. */
+int rkx_batch_status(rkx_batch *b, int id, rkx_stats *st, int *done);
+
+/* Copy an instance's current y (n doubles) to host memory. */
+int rkx_batch_result(rkx_batch *b, int id, double *y);
+
+/* Release an instance's device memory (its id may be reused). */
+int rkx_batch_remove(rkx_batch *b, int id);
+
+/* Device bytes held by the batch's instances. */
+size_t rkx_batch_bytes(const rkx_batch *b);
+
 /* ---- runtime -------------------------------------------------------------- */
 const char *rkx_last_error(void);
 int rkx_device_sync(void);

@@ -17,10 +17,11 @@ from ._lib import (RKX_BRUSS2D, RKX_LV, RKX_LV_CHAIN, RKX_IMPL_D, RKX_IMPL_PIPED
 from .problems import Bruss2D, LV, LVChain, Problem
 from .methods import method, method_table, METHODS
 from .solver import solve, solve_host, solver, Stats
+from .batch import Batch
 
 __all__ = [
     "RKX_BRUSS2D", "RKX_LV", "RKX_LV_CHAIN", "RKX_IMPL_D", "RKX_IMPL_PIPED",
     "RKX_IMPL_PIPEDLS", "RKX_CTRL_DEVICE", "RKX_CTRL_HOST", "RkxError", "lib", "lib_path",
     "Bruss2D", "LV", "LVChain", "Problem", "method", "method_table", "METHODS",
-    "solve", "solve_host", "solver", "Stats",
+    "solve", "solve_host", "solver", "Stats", "Batch",
 ]

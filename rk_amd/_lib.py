@@ -71,6 +71,17 @@ SIGNATURES = [
                             C.POINTER(Stats)]),
     ("rkx_solve_host", C.c_int, [C.POINTER(Ode), C.POINTER(Method), C.c_double, C.c_double,
                                  _dp, _dp, C.c_double, C.POINTER(Opts), C.POINTER(Stats)]),
+    ("rkx_batch_opts_default", None, [C.c_void_p]),
+    ("rkx_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_void_p]),
+    ("rkx_batch_destroy", C.c_int, [C.c_void_p]),
+    ("rkx_batch_footprint", C.c_size_t, [C.POINTER(Ode), C.POINTER(Method), C.c_int]),
+    ("rkx_batch_add", C.c_int, [C.c_void_p, C.POINTER(Ode), C.POINTER(Method), C.c_double,
+                                C.c_double, _dp, C.c_double, C.c_double, C.c_int, C.c_int]),
+    ("rkx_batch_run", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    ("rkx_batch_status", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Stats), C.POINTER(C.c_int)]),
+    ("rkx_batch_result", C.c_int, [C.c_void_p, C.c_int, _dp]),
+    ("rkx_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
+    ("rkx_batch_bytes", C.c_size_t, [C.c_void_p]),
     ("rkx_last_error", C.c_char_p, []),
     ("rkx_device_sync", C.c_int, []),
     ("rkx_solve_footprint", C.c_size_t, [C.POINTER(Ode), C.POINTER(Method), C.c_int]),

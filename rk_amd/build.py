@@ -21,6 +21,7 @@ LIB = os.path.join(HERE, "librkx.so")
 
 SOURCES = [
     "rkx_host.cpp",
+    "rkx_batch.cpp",
     "rkx_methods.cpp",
     "rkx_kernels_d.hip",
     "rkx_kernels_sweep.hip",

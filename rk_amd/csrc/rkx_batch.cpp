@@ -1,0 +1,453 @@
+// Batched solves: many instances of uneven size advanced together, each with
+// its own t, h, accept/reject and step limit (include/rkx.h, rkx_batch_*).
+//
+// Instances with a fused kernel (BRUSS2D, low storage) are grouped by
+// (method pattern, strip width); one launch per group covers the tiles of all
+// of the group's active instances through an (instance, tile) table, and the
+// last tile of each instance runs that instance's step control.  Instances
+// without a fused kernel (full storage) run the stage kernels of the D form.
+// The host polls the instances' states every few batched steps, drops
+// finished instances from the tile tables and lets callers add new ones in
+// between (continuous batching).  An instance's arithmetic depends only on
+// its own data, so its result is bitwise the same alone or in any mix.
+#include "rkx_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <utility>
+#include <vector>
+
+namespace rkx {
+namespace {
+
+constexpr int kBatchCap = 1024;       // instances per batch
+constexpr int kPollSteps = 16;        // batched steps between state polls
+
+struct batch_inst {
+  bool used = false;
+  bool done = false;
+  int storage = RKX_STORAGE_LOW;
+  int pattern = 0;
+  int K = 0;
+  rkx_ode ode{};
+  rkx_method m{};
+  dev_ode dp{};
+  dev_method dm{};
+  sweep_cfg cfg{1, 1};
+  double *y[2] = {nullptr, nullptr};
+  double *w = nullptr, *dy = nullptr, *err = nullptr;
+  size_t bytes = 0;
+  state snap{};
+};
+
+struct tile_group {
+  int pattern = 0, K = 0;
+  std::vector<int2> tiles;
+  int2 *dev = nullptr;
+  size_t dev_cap = 0;
+};
+
+}  // namespace
+}  // namespace rkx
+
+using namespace rkx;
+
+struct rkx_batch {
+  hipStream_t stream = nullptr;
+  rkx_batch_opts opts{};
+  state *st_dev = nullptr;
+  dev_coef *cf_dev = nullptr;
+  dev_method *dm_dev = nullptr;
+  sweep_inst *si_dev = nullptr;
+  state *st_pinned = nullptr;
+  std::vector<batch_inst> inst;
+  std::map<std::pair<int, int>, tile_group> groups;
+  size_t bytes = 0;
+  bool dirty = true;
+  int hi_slot = 0;                      // one past the highest slot ever used
+  pow_service_thread svc;
+};
+
+namespace {
+
+void free_inst(batch_inst &b)
+{
+  for (double *p : {b.y[0], b.y[1], b.w, b.dy, b.err})
+    if (p)
+      (void) hipFree(p);
+  b = batch_inst{};
+}
+
+// strip width and band height of a batched BRUSS2D instance: narrow strips
+// for small grids, and bands of about 64 rows (long enough for the s-1 halo
+// rows to be a small share, short enough that large instances split into
+// many tiles)
+sweep_cfg batch_cfg(const dev_ode &p, const rkx_method &m, const rkx_batch_opts &o)
+{
+  sweep_cfg c;
+  c.waves = (p.N <= 64) ? 1 : 2;
+  if (64 * c.waves <= 2 * m.s)
+    c.waves = 2;
+  c.rows = 64;
+  if (o.tile_waves > 0)
+    c.waves = o.tile_waves;
+  if (o.tile_rows > 0)
+    c.rows = o.tile_rows;
+  return c;
+}
+
+int rebuild_tables(rkx_batch *b)
+{
+  for (auto &kv : b->groups)
+    kv.second.tiles.clear();
+  for (int i = 0; i < b->hi_slot; ++i) {
+    const batch_inst &I = b->inst[static_cast<size_t>(i)];
+    if (!I.used || I.done || I.storage != RKX_STORAGE_LOW)
+      continue;
+    tile_group &g = b->groups[{I.pattern, I.K}];
+    g.pattern = I.pattern;
+    g.K = I.K;
+    const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
+    for (int t = 0; t < si.ntiles; ++t)
+      g.tiles.push_back(make_int2(i, t));
+  }
+  for (auto &kv : b->groups) {
+    tile_group &g = kv.second;
+    if (g.tiles.size() > g.dev_cap) {
+      if (g.dev)
+        (void) hipFree(g.dev);
+      g.dev = nullptr;
+      g.dev_cap = std::max<size_t>(g.tiles.size(), 2 * g.dev_cap);
+      if (hipMalloc(reinterpret_cast<void **>(&g.dev), g.dev_cap * sizeof(int2)) != hipSuccess) {
+        (void) hipGetLastError();
+        g.dev_cap = 0;
+        return fail(RKX_ENOMEM, "rkx_batch: tile table allocation failed");
+      }
+    }
+    if (!g.tiles.empty())
+      RKX_HIP(hipMemcpyAsync(g.dev, g.tiles.data(), g.tiles.size() * sizeof(int2),
+                             hipMemcpyHostToDevice, b->stream));
+  }
+  RKX_HIP(hipStreamSynchronize(b->stream));
+  b->dirty = false;
+  return RKX_OK;
+}
+
+int enqueue_batched_step(rkx_batch *b)
+{
+  for (auto &kv : b->groups) {
+    const tile_group &g = kv.second;
+    if (!g.tiles.empty())
+      RKX_HIP(launch_sweep(g.pattern, g.K, b->si_dev, g.dev, static_cast<int>(g.tiles.size()),
+                           b->stream));
+  }
+  for (int i = 0; i < b->hi_slot; ++i) {
+    batch_inst &I = b->inst[static_cast<size_t>(i)];
+    if (!I.used || I.done || I.storage != RKX_STORAGE_FULL)
+      continue;
+    RKX_HIP(launch_d_step(I.dp, b->cf_dev + i, I.m.s, b->st_dev + i, I.y[0], I.y[1], I.w, I.dy,
+                          I.err, b->stream));
+  }
+  return RKX_OK;
+}
+
+// copy every instance's state back; finish time-outs of the pow service on
+// the host; mark finished instances
+int poll_states(rkx_batch *b, int *newly_done)
+{
+  *newly_done = 0;
+  if (b->hi_slot == 0)
+    return RKX_OK;
+  RKX_HIP(hipMemcpyAsync(b->st_pinned, b->st_dev, sizeof(state) * b->hi_slot,
+                         hipMemcpyDeviceToHost, b->stream));
+  RKX_HIP(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->hi_slot; ++i) {
+    batch_inst &I = b->inst[static_cast<size_t>(i)];
+    if (!I.used)
+      continue;
+    state s = b->st_pinned[i];
+    if (s.pow_wait) {
+      dev_coef hc{};
+      host_finish(s, true, I.dm, hc);
+      RKX_HIP(hipMemcpy(b->cf_dev + i, &hc, sizeof(hc), hipMemcpyHostToDevice));
+      RKX_HIP(hipMemcpy(b->st_dev + i, &s, sizeof(s), hipMemcpyHostToDevice));
+    }
+    I.snap = s;
+    if (s.done && !I.done) {
+      I.done = true;
+      ++*newly_done;
+    }
+  }
+  return RKX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rkx_batch_opts_default(rkx_batch_opts *o)
+{
+  if (!o)
+    return;
+  std::memset(o, 0, sizeof(*o));
+  o->ctrl = RKX_CTRL_DEVICE;
+}
+
+int rkx_batch_create(rkx_batch **out, const rkx_batch_opts *o)
+{
+  if (!out)
+    return fail(RKX_EINVAL, "rkx_batch_create: null output");
+  rkx_batch_opts opts;
+  if (o)
+    opts = *o;
+  else
+    rkx_batch_opts_default(&opts);
+  if (opts.ctrl != RKX_CTRL_DEVICE && opts.ctrl != RKX_CTRL_DEVICE_POW)
+    return fail(RKX_EINVAL, "rkx_batch_create: ctrl must be RKX_CTRL_DEVICE or _DEVICE_POW");
+  rkx_batch *b = new rkx_batch();
+  b->opts = opts;
+  b->stream = static_cast<hipStream_t>(opts.stream);
+  int rc = RKX_OK;
+  if (!b->stream)
+    rc = default_stream(&b->stream);
+  auto dalloc = [&](void **p, size_t bytes) {
+    if (rc)
+      return;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void) hipGetLastError();
+      rc = fail(RKX_ENOMEM, "rkx_batch_create: device allocation failed");
+    }
+  };
+  dalloc(reinterpret_cast<void **>(&b->st_dev), sizeof(state) * kBatchCap);
+  dalloc(reinterpret_cast<void **>(&b->cf_dev), sizeof(dev_coef) * kBatchCap);
+  dalloc(reinterpret_cast<void **>(&b->dm_dev), sizeof(dev_method) * kBatchCap);
+  dalloc(reinterpret_cast<void **>(&b->si_dev), sizeof(sweep_inst) * kBatchCap);
+  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&b->st_pinned), sizeof(state) * kBatchCap,
+                           hipHostMallocDefault) != hipSuccess) {
+    (void) hipGetLastError();
+    rc = fail(RKX_ENOMEM, "rkx_batch_create: pinned allocation failed");
+  }
+  if (!rc && opts.ctrl == RKX_CTRL_DEVICE)
+    rc = b->svc.start(kBatchCap);
+  if (rc) {
+    rkx_batch_destroy(b);
+    return rc;
+  }
+  b->inst.resize(kBatchCap);
+  *out = b;
+  return RKX_OK;
+}
+
+int rkx_batch_destroy(rkx_batch *b)
+{
+  if (!b)
+    return RKX_OK;
+  if (b->stream)
+    (void) hipStreamSynchronize(b->stream);
+  b->svc.stop();
+  for (auto &I : b->inst)
+    if (I.used)
+      free_inst(I);
+  for (auto &kv : b->groups)
+    if (kv.second.dev)
+      (void) hipFree(kv.second.dev);
+  for (void *p : {static_cast<void *>(b->st_dev), static_cast<void *>(b->cf_dev),
+                  static_cast<void *>(b->dm_dev), static_cast<void *>(b->si_dev)})
+    if (p)
+      (void) hipFree(p);
+  if (b->st_pinned)
+    (void) hipHostFree(b->st_pinned);
+  delete b;
+  return RKX_OK;
+}
+
+size_t rkx_batch_footprint(const rkx_ode *p, const rkx_method *m, int storage)
+{
+  if (check_ode(p) || check_method(m))
+    return 0;
+  const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
+  const dev_ode dp = make_dev_ode(*p);
+  if (storage == RKX_STORAGE_AUTO)
+    storage = sweep_supported(dp, *m) ? RKX_STORAGE_LOW : RKX_STORAGE_FULL;
+  if (storage == RKX_STORAGE_LOW)
+    return 2 * vec;
+  return (2 + (m->s - 1) + 2) * vec;
+}
+
+int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0, double te,
+                  const double *y0, double tol, double h0, int step_limit, int storage)
+{
+  if (!b || !y0)
+    return fail(RKX_EINVAL, "rkx_batch_add: null argument");
+  if (int rc = check_ode(p))
+    return rc;
+  if (int rc = check_method(m))
+    return rc;
+  const dev_ode dp = make_dev_ode(*p);
+  const bool fused = sweep_supported(dp, *m);
+  if (storage == RKX_STORAGE_AUTO)
+    storage = fused ? RKX_STORAGE_LOW : RKX_STORAGE_FULL;
+  if (storage == RKX_STORAGE_LOW && !fused)
+    return fail(RKX_EINVAL, "rkx_batch_add: no low-storage (fused) kernel for this problem/method");
+  if (storage != RKX_STORAGE_LOW && storage != RKX_STORAGE_FULL)
+    return fail(RKX_EINVAL, "rkx_batch_add: unknown storage mode");
+  const size_t need = rkx_batch_footprint(p, m, storage);
+  if (b->opts.hbm_budget > 0 && b->bytes + need > b->opts.hbm_budget)
+    return fail(RKX_ELIMIT, "rkx_batch_add: HBM budget exhausted");
+  int slot = -1;
+  for (int i = 0; i < kBatchCap; ++i)
+    if (!b->inst[static_cast<size_t>(i)].used) {
+      slot = i;
+      break;
+    }
+  if (slot < 0)
+    return fail(RKX_ELIMIT, "rkx_batch_add: instance table full");
+  batch_inst &I = b->inst[static_cast<size_t>(slot)];
+  I = batch_inst{};
+  I.storage = storage;
+  I.ode = *p;
+  I.m = *m;
+  I.dp = dp;
+  fill_dev_method(*m, I.dm);
+  const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
+  auto dalloc = [&](double **q, size_t bytes) -> int {
+    if (hipMalloc(reinterpret_cast<void **>(q), bytes) != hipSuccess) {
+      (void) hipGetLastError();
+      return fail(RKX_ENOMEM, "rkx_batch_add: device allocation failed");
+    }
+    return RKX_OK;
+  };
+  int rc = dalloc(&I.y[0], vec);
+  if (!rc)
+    rc = dalloc(&I.y[1], vec);
+  if (!rc && storage == RKX_STORAGE_FULL) {
+    rc = dalloc(&I.w, vec * (m->s - 1));
+    if (!rc)
+      rc = dalloc(&I.dy, vec);
+    if (!rc)
+      rc = dalloc(&I.err, vec);
+  }
+  if (rc) {
+    free_inst(I);
+    return rc;
+  }
+  I.used = true;
+  I.bytes = need;
+
+  state hs{};
+  hs.t = t0;
+  hs.te = te;
+  hs.tol = tol;
+  hs.h = initial_h(te - t0, tol, step_limit, h0);
+  hs.step_limit = step_limit;
+  hs.ord = m->ord;
+  hs.ctrl = b->opts.ctrl;
+  hs.s = m->s;
+  hs.meth = b->dm_dev + slot;
+  hs.coef = b->cf_dev + slot;
+  hs.done = !(t0 < te);
+  if (b->opts.ctrl == RKX_CTRL_DEVICE) {
+    b->svc.reset_slot(slot);
+    hs.mb = b->svc.device_ptr(slot);
+  }
+  dev_coef hc{};
+  premult(I.dm, m->s, hs.h, hc);
+  I.snap = hs;
+  I.done = hs.done;
+  if (storage == RKX_STORAGE_LOW) {
+    I.pattern = sweep_pattern(*m);
+    I.cfg = batch_cfg(dp, *m, b->opts);
+    I.K = I.cfg.waves;
+    const sweep_inst si =
+        make_sweep_inst(dp, *m, I.cfg, b->st_dev + slot, b->cf_dev + slot, I.y[0], I.y[1]);
+    RKX_HIP(hipMemcpyAsync(b->si_dev + slot, &si, sizeof(si), hipMemcpyHostToDevice, b->stream));
+  }
+  RKX_HIP(hipMemcpyAsync(b->dm_dev + slot, &I.dm, sizeof(I.dm), hipMemcpyHostToDevice, b->stream));
+  RKX_HIP(hipMemcpyAsync(b->cf_dev + slot, &hc, sizeof(hc), hipMemcpyHostToDevice, b->stream));
+  RKX_HIP(hipMemcpyAsync(b->st_dev + slot, &hs, sizeof(hs), hipMemcpyHostToDevice, b->stream));
+  RKX_HIP(hipMemcpyAsync(I.y[0], y0, vec, hipMemcpyHostToDevice, b->stream));
+  RKX_HIP(hipStreamSynchronize(b->stream));
+  b->bytes += need;
+  b->hi_slot = std::max(b->hi_slot, slot + 1);
+  b->dirty = true;
+  return slot;
+}
+
+int rkx_batch_run(rkx_batch *b, int max_steps, int *active)
+{
+  if (!b || max_steps < 0)
+    return fail(RKX_EINVAL, "rkx_batch_run: bad argument");
+  auto count_active = [&]() {
+    int a = 0;
+    for (int i = 0; i < b->hi_slot; ++i)
+      a += (b->inst[static_cast<size_t>(i)].used && !b->inst[static_cast<size_t>(i)].done);
+    return a;
+  };
+  int done_steps = 0;
+  while (done_steps < max_steps && count_active() > 0) {
+    if (b->dirty)
+      if (int rc = rebuild_tables(b))
+        return rc;
+    const int k = std::min(kPollSteps, max_steps - done_steps);
+    for (int j = 0; j < k; ++j)
+      if (int rc = enqueue_batched_step(b))
+        return rc;
+    done_steps += k;
+    int newly = 0;
+    if (int rc = poll_states(b, &newly))
+      return rc;
+    if (newly)
+      b->dirty = true;
+  }
+  if (active)
+    *active = count_active();
+  return RKX_OK;
+}
+
+int rkx_batch_status(rkx_batch *b, int id, rkx_stats *st, int *done)
+{
+  if (!b || id < 0 || id >= kBatchCap || !b->inst[static_cast<size_t>(id)].used)
+    return fail(RKX_EINVAL, "rkx_batch_status: unknown instance");
+  const batch_inst &I = b->inst[static_cast<size_t>(id)];
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->steps_acc = I.snap.acc;
+    st->steps_rej = I.snap.rej;
+    st->t = I.snap.t;
+    st->h = I.snap.h;
+    st->resident_bytes = I.bytes;
+  }
+  if (done)
+    *done = I.done ? 1 : 0;
+  return RKX_OK;
+}
+
+int rkx_batch_result(rkx_batch *b, int id, double *y)
+{
+  if (!b || !y || id < 0 || id >= kBatchCap || !b->inst[static_cast<size_t>(id)].used)
+    return fail(RKX_EINVAL, "rkx_batch_result: unknown instance");
+  const batch_inst &I = b->inst[static_cast<size_t>(id)];
+  RKX_HIP(hipStreamSynchronize(b->stream));
+  state s;
+  RKX_HIP(hipMemcpy(&s, b->st_dev + id, sizeof(state), hipMemcpyDeviceToHost));
+  RKX_HIP(hipMemcpy(y, I.y[s.cur], static_cast<size_t>(I.ode.n) * sizeof(double),
+                    hipMemcpyDeviceToHost));
+  return RKX_OK;
+}
+
+int rkx_batch_remove(rkx_batch *b, int id)
+{
+  if (!b || id < 0 || id >= kBatchCap || !b->inst[static_cast<size_t>(id)].used)
+    return fail(RKX_EINVAL, "rkx_batch_remove: unknown instance");
+  RKX_HIP(hipStreamSynchronize(b->stream));
+  batch_inst &I = b->inst[static_cast<size_t>(id)];
+  b->bytes -= I.bytes;
+  free_inst(I);
+  b->dirty = true;
+  return RKX_OK;
+}
+
+size_t rkx_batch_bytes(const rkx_batch *b) { return b ? b->bytes : 0; }
+
+}  // extern "C"

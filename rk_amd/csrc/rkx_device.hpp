@@ -187,11 +187,14 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 // Folds each thread's (qmax, zero) into the step accumulators and runs the
-// control in the last workgroup to finish.  Must be called by all threads of
-// the block.  red must hold blockDim.x/64 doubles of LDS.
+// control in the last of the `nblocks` workgroups that work on this state to
+// finish (nblocks = 0: the whole grid).  Must be called by all threads of the
+// block.  red must hold blockDim.x/64 doubles of LDS.
 __device__ inline void block_finish(state *st, double qmax, unsigned int zero, double *red,
-                                    unsigned int *zred)
+                                    unsigned int *zred, unsigned int nblocks = 0u)
 {
+  if (nblocks == 0u)
+    nblocks = gridDim.x * gridDim.y * gridDim.z;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
@@ -215,7 +218,7 @@ __device__ inline void block_finish(state *st, double qmax, unsigned int zero, d
       atomicOr(&st->zero_div, 1u);
     __threadfence();
     const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
-    is_last = (prev == gridDim.x * gridDim.y * gridDim.z - 1u);
+    is_last = (prev == nblocks - 1u);
   }
   __syncthreads();
   if (is_last && threadIdx.x == 0) {

@@ -51,9 +51,9 @@ dev_ode make_dev_ode(const rkx_ode &p)
 }
 
 namespace {
-
 hipStream_t g_stream = nullptr;
 std::mutex g_init_mu;
+}  // namespace
 
 int default_stream(hipStream_t *out)
 {
@@ -125,6 +125,8 @@ double initial_h(double H, double tol, int limit, double h0)
                    : H / static_cast<double>(limit);
 }
 
+namespace {
+
 // All device memory of one solve, released on scope exit.
 struct solve_mem {
   std::vector<void *> ptrs;
@@ -158,62 +160,68 @@ struct graph_holder {
   }
 };
 
-// Host pow service: answers pow(x, y) requests that the device step control
-// posts in a mailbox of fine-grained pinned host memory, with the C library's
-// pow -- the one libm call of the reference's step control (tools.h:226), so
-// the device reproduces the reference's step sizes bitwise.  One busy-polling
-// thread per solve.
-class pow_service_thread {
- public:
-  int start()
-  {
-    void *p = nullptr;
-    if (hipHostMalloc(&p, sizeof(pow_mailbox), hipHostMallocCoherent | hipHostMallocMapped) !=
-        hipSuccess) {
-      (void) hipGetLastError();
-      return fail(RKX_ENOMEM, "hipHostMalloc of the pow mailbox failed");
-    }
-    mb_ = static_cast<pow_mailbox *>(p);
-    std::memset(mb_, 0, sizeof(*mb_));
-    void *dp = nullptr;
-    RKX_HIP(hipHostGetDevicePointer(&dp, p, 0));
-    dev_ = static_cast<pow_mailbox *>(dp);
-    stop_.store(false);
-    th_ = std::thread([this] { run(); });
-    return RKX_OK;
-  }
-  pow_mailbox *device_ptr() const { return dev_; }
-  ~pow_service_thread()
-  {
-    stop_.store(true);
-    if (th_.joinable())
-      th_.join();
-    if (mb_)
-      (void) hipHostFree(mb_);
-  }
+}  // namespace
 
- private:
-  void run()
-  {
-    unsigned long long handled = 0;
-    while (!stop_.load(std::memory_order_relaxed)) {
-      const unsigned long long r = __atomic_load_n(&mb_->req, __ATOMIC_ACQUIRE);
-      if (r != handled) {
-        const double x = *reinterpret_cast<volatile double *>(&mb_->x);
-        const double y = *reinterpret_cast<volatile double *>(&mb_->y);
-        *reinterpret_cast<volatile double *>(&mb_->p) = std::pow(x, y);
-        __atomic_store_n(&mb_->resp, r, __ATOMIC_RELEASE);
-        handled = r;
-      } else {
-        __builtin_ia32_pause();
+int pow_service_thread::start(int nslots)
+{
+  void *p = nullptr;
+  const size_t bytes = sizeof(pow_mailbox) * static_cast<size_t>(nslots);
+  if (hipHostMalloc(&p, bytes, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
+    (void) hipGetLastError();
+    return fail(RKX_ENOMEM, "hipHostMalloc of the pow mailbox failed");
+  }
+  mb_ = static_cast<pow_mailbox *>(p);
+  std::memset(mb_, 0, bytes);
+  nslots_ = nslots;
+  handled_.assign(static_cast<size_t>(nslots), 0ull);
+  void *dp = nullptr;
+  RKX_HIP(hipHostGetDevicePointer(&dp, p, 0));
+  dev_ = static_cast<pow_mailbox *>(dp);
+  stop_.store(false);
+  th_ = std::thread([this] { run(); });
+  return RKX_OK;
+}
+
+void pow_service_thread::stop()
+{
+  stop_.store(true);
+  if (th_.joinable())
+    th_.join();
+  if (mb_)
+    (void) hipHostFree(mb_);
+  mb_ = nullptr;
+  dev_ = nullptr;
+}
+
+void pow_service_thread::reset_slot(int i)
+{
+  // A slot handed to a new solve starts from request number 0 again.  The
+  // service thread sees req change and answers once more (a harmless extra
+  // answer nobody waits for), then follows the new solve's numbers.
+  __atomic_store_n(&mb_[i].req, 0ull, __ATOMIC_RELEASE);
+  __atomic_store_n(&mb_[i].resp, 0ull, __ATOMIC_RELEASE);
+}
+
+void pow_service_thread::run()
+{
+  while (!stop_.load(std::memory_order_relaxed)) {
+    bool any = false;
+    for (int i = 0; i < nslots_; ++i) {
+      pow_mailbox *m = mb_ + i;
+      const unsigned long long r = __atomic_load_n(&m->req, __ATOMIC_ACQUIRE);
+      if (r != handled_[static_cast<size_t>(i)]) {
+        const double x = *reinterpret_cast<volatile double *>(&m->x);
+        const double y = *reinterpret_cast<volatile double *>(&m->y);
+        *reinterpret_cast<volatile double *>(&m->p) = std::pow(x, y);
+        __atomic_store_n(&m->resp, r, __ATOMIC_RELEASE);
+        handled_[static_cast<size_t>(i)] = r;
+        any = true;
       }
     }
+    if (!any)
+      __builtin_ia32_pause();
   }
-  pow_mailbox *mb_ = nullptr;
-  pow_mailbox *dev_ = nullptr;
-  std::atomic<bool> stop_{false};
-  std::thread th_;
-};
+}
 
 // Host half of the step control: ctrl_decide/ctrl_finish with the C library's
 // pow, then the premultiplied coefficients of the next step.
@@ -230,8 +238,6 @@ void host_finish(state &st, bool decided, const dev_method &dm, dev_coef &hc)
   ctrl_finish(st, need, p);
   premult(dm, st.s, st.h, hc);
 }
-
-}  // namespace
 
 }  // namespace rkx
 
@@ -500,11 +506,25 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     cfg.rows = o.tile_rows;
   if (o.tile_waves > 0)
     cfg.waves = o.tile_waves;
+  if (cfg.waves < 1 || cfg.waves > 4 || cfg.rows < 1 || 64 * cfg.waves <= 2 * s)
+    return fail(RKX_EINVAL, "rkx_solve: tile_waves must be 1..4 (and wider than 2s), "
+                            "tile_rows >= 1");
+  sweep_inst *inst_dev = nullptr;
+  int ntiles = 0;
+  const int pattern = sweep_pattern(*m);
+  if (impl != RKX_IMPL_D) {
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&inst_dev), sizeof(sweep_inst)))
+      return rc;
+    const sweep_inst hi = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
+    ntiles = hi.ntiles;
+    RKX_HIP(hipMemcpyAsync(inst_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));
+    RKX_HIP(hipStreamSynchronize(stream));
+  }
 
   auto enqueue_step = [&]() -> hipError_t {
     if (impl == RKX_IMPL_D)
       return launch_d_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], w, dy, err, stream);
-    return launch_sweep_step(dp, *m, cf_dev, st_dev, ybuf[0], ybuf[1], cfg, stream);
+    return launch_sweep(pattern, cfg.waves, inst_dev, nullptr, ntiles, stream);
   };
 
   state *hs_pinned = nullptr;   // [0], [1]: snapshots of two chunks in flight

@@ -3,9 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/rkx.h"
 
@@ -145,6 +148,39 @@ __host__ __device__ inline void ctrl_finish(state &st, bool used_pow, double p)
   st.done = !((st.t < st.te) && (st.step_limit <= 0 || attempts < st.step_limit));
 }
 
+// ---- host runtime helpers (rkx_host.cpp) ------------------------------------
+int check_ode(const rkx_ode *p);
+int check_method(const rkx_method *m);
+void fill_dev_method(const rkx_method &m, dev_method &d);
+double initial_h(double H, double tol, int limit, double h0);   // src/tools.c:25-36
+int default_stream(hipStream_t *out);
+// the host half of the step control (ctrl_decide/ctrl_finish with the C
+// library's pow) and the next step's premultiplied coefficients
+void host_finish(state &st, bool decided, const dev_method &dm, dev_coef &hc);
+
+// Host pow service: answers pow(x, y) requests that device step controls post
+// in mailboxes of fine-grained pinned host memory (one slot per concurrent
+// solve), with the C library's pow -- the one libm call of the reference's
+// step control (tools.h:226) -- so device-side control reproduces the
+// reference's step sizes bitwise.  One busy-polling host thread.
+class pow_service_thread {
+ public:
+  int start(int nslots = 1);
+  void stop();
+  void reset_slot(int i);
+  pow_mailbox *device_ptr(int slot = 0) const { return dev_ ? dev_ + slot : nullptr; }
+  ~pow_service_thread() { stop(); }
+
+ private:
+  void run();
+  pow_mailbox *mb_ = nullptr;
+  pow_mailbox *dev_ = nullptr;
+  int nslots_ = 0;
+  std::vector<unsigned long long> handled_;
+  std::atomic<bool> stop_{false};
+  std::thread th_;
+};
+
 // Kernel launchers (rkx_kernels_*.hip).  All return hipError_t of the launch.
 hipError_t launch_eval_rng(const dev_ode &p, int i, int j, const double *y, double *f,
                            hipStream_t s);
@@ -156,10 +192,28 @@ struct sweep_cfg {
   int waves;     // wavefronts per workgroup (strip width 64*waves lanes)
   int rows;      // band height in grid rows
 };
+// One BRUSS2D instance as the fused kernel sees it (device memory).  A launch
+// covers a table of (instance, tile) pairs, so one kernel steps many
+// instances, each with its own state, coefficients and step control.
+struct sweep_inst {
+  int N;                 // grid size
+  int R;                 // band height (rows)
+  int OC;                // output columns per strip
+  int nstrips, nbands;
+  int ntiles;            // nstrips * nbands: workgroups that share this state per step
+  double coef;           // (alpha*(N-1))*(N-1)
+  double *y0, *y1;       // the two y buffers
+  state *st;
+  const dev_coef *cf;
+};
+
 bool sweep_supported(const dev_ode &p, const rkx_method &m);
-hipError_t launch_sweep_step(const dev_ode &p, const rkx_method &m, const dev_coef *cf,
-                             state *st, double *y0, double *y1, const sweep_cfg &cfg,
-                             hipStream_t stream);
+int sweep_pattern(const rkx_method &m);   // compiled zero pattern (0: none)
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m);
+sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
+                           const dev_coef *cf, double *y0, double *y1);
+// tiles == nullptr: a single instance (insts[0]) with ntiles tiles
+hipError_t launch_sweep(int pattern, int K, const sweep_inst *insts, const int2 *tiles,
+                        int ntiles, hipStream_t stream);
 
 }  // namespace rkx

@@ -111,9 +111,8 @@ __device__ __forceinline__ int reflect(int k, int N)
 }
 
 template <class P, int K>
-__global__ __launch_bounds__(128 * K) void k_sweep(dev_ode p, const dev_coef *__restrict__ cf,
-                                                   state *st, double *y0, double *y1, int R,
-                                                   int OC, int nstrips, int nbands)
+__global__ __launch_bounds__(128 * K) void k_sweep(const sweep_inst *__restrict__ insts,
+                                                   const int2 *__restrict__ tiles, int ntiles)
 {
   constexpr int s = P::s;
   constexpr int L = 64 * K;          // columns per strip
@@ -125,24 +124,29 @@ __global__ __launch_bounds__(128 * K) void k_sweep(dev_ode p, const dev_coef *__
   __shared__ double red[2 * K];
   __shared__ unsigned int zred[2 * K];
 
-  if (st->done | st->pow_wait)
-    return;                                      // uniform over the whole grid
-  const int cur = st->cur;
-  const double *__restrict__ y = cur ? y1 : y0;
-  double *__restrict__ yn = cur ? y0 : y1;
-  const int N = p.N;
-  const int N2 = N + N;
-  const double coef = p.coef;
-
   // XCD-aware tile order: consecutive workgroups are dispatched round-robin
   // over the 8 XCDs; give each XCD a contiguous run of tiles (adjacent bands
   // of a strip share halo rows in that XCD's L2).
-  const int T = gridDim.x;
+  const int T = ntiles;
   const int b = blockIdx.x;
   const int x = b & 7, q = T >> 3, rm = T & 7;
   const int tile = x * q + min(x, rm) + (b >> 3);
-  const int strip = tile / nbands;
-  const int band = tile - strip * nbands;
+  // tile -> (instance, tile within the instance); no table: one instance
+  const int2 td = tiles ? tiles[tile] : make_int2(0, tile);
+  const sweep_inst &I = insts[td.x];
+  state *st = I.st;
+  if (st->done | st->pow_wait)
+    return;                                      // uniform over the instance's tiles
+  const int cur = st->cur;
+  const double *__restrict__ y = cur ? I.y1 : I.y0;
+  double *__restrict__ yn = cur ? I.y0 : I.y1;
+  const int N = I.N;
+  const int N2 = N + N;
+  const double coef = I.coef;
+  const int R = I.R, OC = I.OC;
+  const dev_coef *__restrict__ cf = I.cf;
+  const int strip = td.y / I.nbands;
+  const int band = td.y - strip * I.nbands;
 
   const int var = (threadIdx.x >= L) ? 1 : 0;   // wave-uniform: 0 = u, 1 = v
   const int cl = threadIdx.x - var * L;         // column lane within the strip
@@ -315,23 +319,14 @@ __global__ __launch_bounds__(128 * K) void k_sweep(dev_ode p, const dev_coef *__
     rows(std::integral_constant<int, 0>{});
   else
     rows(std::integral_constant<int, 1>{});
-  block_finish(st, qmax, zero, red, zred);
+  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.ntiles));
 }
 
 template <class P, int K>
-hipError_t launch_one(const dev_ode &p, const dev_coef *cf, state *st, double *y0, double *y1,
-                      int R, hipStream_t stream)
+hipError_t launch_one(const sweep_inst *insts, const int2 *tiles, int ntiles, hipStream_t stream)
 {
-  constexpr int s = P::s;
-  constexpr int L = 64 * K;
-  const int N = p.N;
-  const int oc_max = L - 2 * s;
-  const int nstrips = (N + oc_max - 1) / oc_max;
-  const int OC = (N + nstrips - 1) / nstrips;
-  const int nbands = (N + R - 1) / R;
-  const int grid = nstrips * nbands;
-  hipLaunchKernelGGL((k_sweep<P, K>), dim3(grid), dim3(2 * L), 0, stream, p, cf, st, y0, y1, R,
-                     OC, nstrips, nbands);
+  hipLaunchKernelGGL((k_sweep<P, K>), dim3(ntiles), dim3(128 * K), 0, stream, insts, tiles,
+                     ntiles);
   return hipGetLastError();
 }
 
@@ -373,14 +368,14 @@ int pattern_of(const rkx_method &m)
 }
 
 template <class P>
-hipError_t launch_pat(const dev_ode &p, const dev_coef *cf, state *st, double *y0, double *y1,
-                      const sweep_cfg &c, hipStream_t stream)
+hipError_t launch_pat(int K, const sweep_inst *insts, const int2 *tiles, int ntiles,
+                      hipStream_t stream)
 {
-  switch (c.waves) {
-  case 1: return launch_one<P, 1>(p, cf, st, y0, y1, c.rows, stream);
-  case 2: return launch_one<P, 2>(p, cf, st, y0, y1, c.rows, stream);
-  case 3: return launch_one<P, 3>(p, cf, st, y0, y1, c.rows, stream);
-  default: return launch_one<P, 4>(p, cf, st, y0, y1, c.rows, stream);
+  switch (K) {
+  case 1: return launch_one<P, 1>(insts, tiles, ntiles, stream);
+  case 2: return launch_one<P, 2>(insts, tiles, ntiles, stream);
+  case 3: return launch_one<P, 3>(insts, tiles, ntiles, stream);
+  default: return launch_one<P, 4>(insts, tiles, ntiles, stream);
   }
 }
 
@@ -456,17 +451,41 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
   return c;
 }
 
-hipError_t launch_sweep_step(const dev_ode &p, const rkx_method &m, const dev_coef *cf,
-                             state *st, double *y0, double *y1, const sweep_cfg &c,
-                             hipStream_t stream)
+// Geometry of one instance's tiles for strips of 64*K columns and bands of R rows.
+sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
+                           const dev_coef *cf, double *y0, double *y1)
 {
-  switch (pattern_of(m)) {
-  case 1: return launch_pat<pat_dopri54>(p, cf, st, y0, y1, c, stream);
-  case 2: return launch_pat<pat_verner65>(p, cf, st, y0, y1, c, stream);
-  case 3: return launch_pat<pat_rkf23>(p, cf, st, y0, y1, c, stream);
-  case 4: return launch_pat<pat_heun2>(p, cf, st, y0, y1, c, stream);
-  case 5: return launch_pat<pat_rk4>(p, cf, st, y0, y1, c, stream);
-  case 6: return launch_pat<pat_ssprk3>(p, cf, st, y0, y1, c, stream);
+  sweep_inst I{};
+  const int L = 64 * c.waves;
+  const int oc_max = L - 2 * m.s;
+  I.N = p.N;
+  I.R = c.rows;
+  I.nstrips = (p.N + oc_max - 1) / oc_max;
+  I.OC = (p.N + I.nstrips - 1) / I.nstrips;
+  I.nbands = (p.N + c.rows - 1) / c.rows;
+  I.ntiles = I.nstrips * I.nbands;
+  I.coef = p.coef;
+  I.y0 = y0;
+  I.y1 = y1;
+  I.st = st;
+  I.cf = cf;
+  return I;
+}
+
+int sweep_pattern(const rkx_method &m) { return pattern_of(m); }
+
+hipError_t launch_sweep(int pattern, int K, const sweep_inst *insts, const int2 *tiles,
+                        int ntiles, hipStream_t stream)
+{
+  if (ntiles <= 0)
+    return hipSuccess;
+  switch (pattern) {
+  case 1: return launch_pat<pat_dopri54>(K, insts, tiles, ntiles, stream);
+  case 2: return launch_pat<pat_verner65>(K, insts, tiles, ntiles, stream);
+  case 3: return launch_pat<pat_rkf23>(K, insts, tiles, ntiles, stream);
+  case 4: return launch_pat<pat_heun2>(K, insts, tiles, ntiles, stream);
+  case 5: return launch_pat<pat_rk4>(K, insts, tiles, ntiles, stream);
+  case 6: return launch_pat<pat_ssprk3>(K, insts, tiles, ntiles, stream);
   default: return hipErrorInvalidValue;
   }
 }
