@@ -191,6 +191,18 @@ int rkx_batch_remove(rkx_batch *b, int id);
 /* Device bytes held by the batch's instances. */
 size_t rkx_batch_bytes(const rkx_batch *b);
 
+/* ---- ensemble of independent LV pairs (no reference counterpart; the
+ * reference's LV has one shared h for identical pairs, lv.c:31-37) ---------
+ * `count` systems x' = c0 x - d0 x y, y' = d1 x y - c1 y, constants
+ * params_dev[4k..4k+3] = (c0, d0, d1, c1), start y0_dev[2k..2k+1], each
+ * integrated from t0 to te with its own step size, accept/reject and the
+ * stage arithmetic of seq D; the step control uses the device pow (as
+ * RKX_CTRL_DEVICE_POW).  h0 > 0 is the initial step of every system.  Writes
+ * y_dev[2k..2k+1] and the per-system accepted/rejected counts (device). */
+int rkx_lv_ensemble(int count, const double *params_dev, const double *y0_dev, double *y_dev,
+                    double t0, double te, const rkx_method *m, double tol, double h0,
+                    int step_limit, int *acc_dev, int *rej_dev, void *stream);
+
 /* ---- runtime -------------------------------------------------------------- */
 const char *rkx_last_error(void);
 int rkx_device_sync(void);

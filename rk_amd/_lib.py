@@ -82,6 +82,9 @@ SIGNATURES = [
     ("rkx_batch_result", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("rkx_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
     ("rkx_batch_bytes", C.c_size_t, [C.c_void_p]),
+    ("rkx_lv_ensemble", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                  C.c_double, C.POINTER(Method), C.c_double, C.c_double, C.c_int,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rkx_last_error", C.c_char_p, []),
     ("rkx_device_sync", C.c_int, []),
     ("rkx_solve_footprint", C.c_size_t, [C.POINTER(Ode), C.POINTER(Method), C.c_int]),

@@ -25,6 +25,8 @@ SOURCES = [
     "rkx_methods.cpp",
     "rkx_kernels_d.hip",
     "rkx_kernels_sweep.hip",
+    "rkx_kernels_lv.hip",
+    "rkx_kernels_chain.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

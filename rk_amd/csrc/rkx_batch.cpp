@@ -1,8 +1,9 @@
 // Batched solves: many instances of uneven size advanced together, each with
 // its own t, h, accept/reject and step limit (include/rkx.h, rkx_batch_*).
 //
-// Instances with a fused kernel (BRUSS2D, low storage) are grouped by
-// (method pattern, strip width); one launch per group covers the tiles of all
+// Instances with a fused kernel (BRUSS2D row sweep, LV chain segments; low
+// storage) are grouped by (method pattern, kernel and strip width); one
+// launch per group covers the tiles of all
 // of the group's active instances through an (instance, tile) table, and the
 // last tile of each instance runs that instance's step control.  Instances
 // without a fused kernel (full storage) run the stage kernels of the D form.
@@ -29,7 +30,7 @@ struct batch_inst {
   bool done = false;
   int storage = RKX_STORAGE_LOW;
   int pattern = 0;
-  int K = 0;
+  int K = 0;                            // strip waves of the row sweep; 0: chain kernel
   rkx_ode ode{};
   rkx_method m{};
   dev_ode dp{};
@@ -60,6 +61,7 @@ struct rkx_batch {
   dev_coef *cf_dev = nullptr;
   dev_method *dm_dev = nullptr;
   sweep_inst *si_dev = nullptr;
+  chain_inst *ci_dev = nullptr;
   state *st_pinned = nullptr;
   std::vector<batch_inst> inst;
   std::map<std::pair<int, int>, tile_group> groups;
@@ -108,8 +110,10 @@ int rebuild_tables(rkx_batch *b)
     tile_group &g = b->groups[{I.pattern, I.K}];
     g.pattern = I.pattern;
     g.K = I.K;
-    const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
-    for (int t = 0; t < si.ntiles; ++t)
+    const int nt =
+        I.K == 0 ? make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).ntiles
+                 : make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).ntiles;
+    for (int t = 0; t < nt; ++t)
       g.tiles.push_back(make_int2(i, t));
   }
   for (auto &kv : b->groups) {
@@ -138,7 +142,12 @@ int enqueue_batched_step(rkx_batch *b)
 {
   for (auto &kv : b->groups) {
     const tile_group &g = kv.second;
-    if (!g.tiles.empty())
+    if (g.tiles.empty())
+      continue;
+    if (g.K == 0)
+      RKX_HIP(launch_chain(g.pattern, b->ci_dev, g.dev, static_cast<int>(g.tiles.size()),
+                           b->stream));
+    else
       RKX_HIP(launch_sweep(g.pattern, g.K, b->si_dev, g.dev, static_cast<int>(g.tiles.size()),
                            b->stream));
   }
@@ -223,6 +232,7 @@ int rkx_batch_create(rkx_batch **out, const rkx_batch_opts *o)
   dalloc(reinterpret_cast<void **>(&b->cf_dev), sizeof(dev_coef) * kBatchCap);
   dalloc(reinterpret_cast<void **>(&b->dm_dev), sizeof(dev_method) * kBatchCap);
   dalloc(reinterpret_cast<void **>(&b->si_dev), sizeof(sweep_inst) * kBatchCap);
+  dalloc(reinterpret_cast<void **>(&b->ci_dev), sizeof(chain_inst) * kBatchCap);
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&b->st_pinned), sizeof(state) * kBatchCap,
                            hipHostMallocDefault) != hipSuccess) {
     (void) hipGetLastError();
@@ -253,7 +263,8 @@ int rkx_batch_destroy(rkx_batch *b)
     if (kv.second.dev)
       (void) hipFree(kv.second.dev);
   for (void *p : {static_cast<void *>(b->st_dev), static_cast<void *>(b->cf_dev),
-                  static_cast<void *>(b->dm_dev), static_cast<void *>(b->si_dev)})
+                  static_cast<void *>(b->dm_dev), static_cast<void *>(b->si_dev),
+                  static_cast<void *>(b->ci_dev)})
     if (p)
       (void) hipFree(p);
   if (b->st_pinned)
@@ -269,7 +280,8 @@ size_t rkx_batch_footprint(const rkx_ode *p, const rkx_method *m, int storage)
   const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
   const dev_ode dp = make_dev_ode(*p);
   if (storage == RKX_STORAGE_AUTO)
-    storage = sweep_supported(dp, *m) ? RKX_STORAGE_LOW : RKX_STORAGE_FULL;
+    storage = (sweep_supported(dp, *m) || chain_supported(dp, *m)) ? RKX_STORAGE_LOW
+                                                                     : RKX_STORAGE_FULL;
   if (storage == RKX_STORAGE_LOW)
     return 2 * vec;
   return (2 + (m->s - 1) + 2) * vec;
@@ -285,7 +297,8 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
   if (int rc = check_method(m))
     return rc;
   const dev_ode dp = make_dev_ode(*p);
-  const bool fused = sweep_supported(dp, *m);
+  const bool chain = chain_supported(dp, *m);
+  const bool fused = chain || sweep_supported(dp, *m);
   if (storage == RKX_STORAGE_AUTO)
     storage = fused ? RKX_STORAGE_LOW : RKX_STORAGE_FULL;
   if (storage == RKX_STORAGE_LOW && !fused)
@@ -355,7 +368,13 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
   premult(I.dm, m->s, hs.h, hc);
   I.snap = hs;
   I.done = hs.done;
-  if (storage == RKX_STORAGE_LOW) {
+  if (storage == RKX_STORAGE_LOW && chain) {
+    I.pattern = sweep_pattern(*m);
+    I.K = 0;
+    const chain_inst ci =
+        make_chain_inst(dp, *m, b->st_dev + slot, b->cf_dev + slot, I.y[0], I.y[1]);
+    RKX_HIP(hipMemcpyAsync(b->ci_dev + slot, &ci, sizeof(ci), hipMemcpyHostToDevice, b->stream));
+  } else if (storage == RKX_STORAGE_LOW) {
     I.pattern = sweep_pattern(*m);
     I.cfg = batch_cfg(dp, *m, b->opts);
     I.K = I.cfg.waves;

@@ -401,7 +401,9 @@ size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
     return 0;
   const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
   const size_t fixed = sizeof(state) + sizeof(dev_method) + sizeof(dev_coef);
-  if (impl == RKX_IMPL_D)
+  const dev_ode dp = make_dev_ode(*p);
+  const bool fused = sweep_supported(dp, *m) || chain_supported(dp, *m);
+  if (impl == RKX_IMPL_D || !fused)
     return (2 + (m->s - 1) + 2) * vec + fixed;   // y x2, w_1..w_{s-1}, dy, err
   return 2 * vec + fixed;                        // y x2
 }
@@ -435,8 +437,12 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   const dev_ode dp = make_dev_ode(*p);
   const int s = m->s;
   int impl = o.impl;
-  if (impl != RKX_IMPL_D && !sweep_supported(dp, *m))
-    impl = RKX_IMPL_D;   // the fused sweep covers BRUSS2D; others use the stage kernels
+  // fused forms: the row sweep for BRUSS2D, the segment kernel for the LV
+  // chain; anything else (LV pairs, tables without a compiled pattern) runs
+  // the stage kernels of D
+  const bool chain = impl != RKX_IMPL_D && chain_supported(dp, *m);
+  if (impl != RKX_IMPL_D && !chain && !sweep_supported(dp, *m))
+    impl = RKX_IMPL_D;
   const size_t n = static_cast<size_t>(p->n);
   const size_t vec = n * sizeof(double);
 
@@ -500,7 +506,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   RKX_HIP(hipMemcpyAsync(ybuf[0], y0_dev, vec, hipMemcpyDeviceToDevice, stream));
 
   sweep_cfg cfg{1, 1};
-  if (impl != RKX_IMPL_D)
+  if (impl != RKX_IMPL_D && !chain)
     cfg = sweep_default_cfg(dp, *m);
   if (o.tile_rows > 0)
     cfg.rows = o.tile_rows;
@@ -510,9 +516,17 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     return fail(RKX_EINVAL, "rkx_solve: tile_waves must be 1..4 (and wider than 2s), "
                             "tile_rows >= 1");
   sweep_inst *inst_dev = nullptr;
+  chain_inst *chain_dev = nullptr;
   int ntiles = 0;
   const int pattern = sweep_pattern(*m);
-  if (impl != RKX_IMPL_D) {
+  if (chain) {
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&chain_dev), sizeof(chain_inst)))
+      return rc;
+    const chain_inst hi = make_chain_inst(dp, *m, st_dev, cf_dev, ybuf[0], ybuf[1]);
+    ntiles = hi.ntiles;
+    RKX_HIP(hipMemcpyAsync(chain_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));
+    RKX_HIP(hipStreamSynchronize(stream));
+  } else if (impl != RKX_IMPL_D) {
     if (int rc = mem.alloc(reinterpret_cast<void **>(&inst_dev), sizeof(sweep_inst)))
       return rc;
     const sweep_inst hi = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
@@ -524,6 +538,8 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   auto enqueue_step = [&]() -> hipError_t {
     if (impl == RKX_IMPL_D)
       return launch_d_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], w, dy, err, stream);
+    if (chain)
+      return launch_chain(pattern, chain_dev, nullptr, ntiles, stream);
     return launch_sweep(pattern, cfg.waves, inst_dev, nullptr, ntiles, stream);
   };
 

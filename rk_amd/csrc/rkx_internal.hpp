@@ -216,4 +216,21 @@ sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cf
 hipError_t launch_sweep(int pattern, int K, const sweep_inst *insts, const int2 *tiles,
                         int ntiles, hipStream_t stream);
 
+// One LV chain instance as the fused chain kernel sees it (device memory).
+struct chain_inst {
+  int cells;             // M
+  int tile_out;          // output cells per segment
+  int ntiles;            // segments (workgroups per step)
+  double c0, d0, d1, c1, diff;
+  double *y0, *y1;
+  state *st;
+  const dev_coef *cf;
+};
+
+bool chain_supported(const dev_ode &p, const rkx_method &m);
+chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, const dev_coef *cf,
+                           double *y0, double *y1);
+hipError_t launch_chain(int pattern, const chain_inst *insts, const int2 *tiles, int ntiles,
+                        hipStream_t stream);
+
 }  // namespace rkx

@@ -95,6 +95,8 @@ def test_full_storage_and_mixed_kinds(rk):
     cases = [(*bruss_instance(30, 2e-3, 0), "DOPRI54", "low"),
              (*bruss_instance(30, 2e-3, 0), "DOPRI54", "full"),
              (*chain_instance(3000, 0), "DOPRI54", "auto"),
+             (*chain_instance(3000, 0), "DOPRI54", "full"),
+             (*chain_instance(2100, 1), "VERNER65", "low"),
              (*bruss_instance(20, 2e-3, 4), "VERNER65", "auto"),
              (*bruss_instance(12, 2e-3, 5), "DOPRI87", "auto")]
     with rk.Batch() as b:

@@ -137,6 +137,44 @@ def test_golden_dump_host_control(rk, name, impl):
     assert np.array_equal(y, z["Y"][-1])
 
 
+CHAIN = [(3, "DOPRI54"), (7, "RKF23"), (1010, "DOPRI54"), (1011, "DOPRI54"),
+         (2500, "VERNER65"), (4100, "RK4"), (3000, "SSPRK3"), (999, "HEUN2")]
+
+
+@pytest.mark.parametrize("M,meth", CHAIN)
+@pytest.mark.parametrize("impl", ["D", "PipeDls"])
+def test_chain_host_control_bitwise(rk, orc, M, meth, impl):
+    """The LV chain through the stage kernels and the fused segment kernel:
+    segments of 1024 cells minus an s-cell halo on each side, so the sizes
+    cover a chain shorter than the halo, exactly one segment, one cell more,
+    and several segments; every attempted step equals the oracle's bitwise."""
+    diff = 0.37 if M < 10 else 1e-3
+    p = rk.LVChain(M, diff=diff)
+    po = orc.make_ode("lvchain", M, diff=diff)
+    y0 = p.ode_start_seeded(3, M, 0.0)
+    m = rk.method(meth)
+    tol = 1e-7 if m.has_emb else 0.0
+    te = 0.5 if m.has_emb else 0.05
+    yo, so, tro = orc.solve(po, meth, 0.0, te, y0, tol, h0=1e-3, trace_cap=20000)
+    y, st, tr = run_gpu(rk, p, meth, 0.0, te, y0, tol, h0=1e-3, impl=impl, ctrl="host",
+                        trace_cap=20000)
+    assert (st.steps_acc, st.steps_rej) == (so["acc"], so["rej"])
+    assert np.array_equal(tr, tro)
+    assert np.array_equal(y, yo)
+
+
+def test_chain_device_control_bitwise(rk, orc):
+    M = 5000
+    p = rk.LVChain(M)
+    po = orc.make_ode("lvchain", M)
+    y0 = p.ode_start_seeded(1, 0, 0.0)
+    yo, so, _ = orc.solve(po, "DOPRI54", 0.0, 2.0, y0, 1e-8, h0=1e-4)
+    y, st, _ = run_gpu(rk, p, "DOPRI54", 0.0, 2.0, y0, 1e-8, h0=1e-4, impl="PipeDls")
+    assert (st.steps_acc, st.steps_rej) == (so["acc"], so["rej"])
+    assert np.array_equal(y, yo)
+    assert st.resident_bytes < 3 * p.n * 8         # two vectors: the low-storage form
+
+
 # ---- device step control ----------------------------------------------------------
 
 KAT_CPU = [pytest.param(c, id=c["name"]) for c in KAT if not c.get("gpu_only")]
