@@ -75,6 +75,11 @@ int rkx_ode_start_seeded(const rkx_ode *p, unsigned long long seed, int instance
 int rkx_eval_rng_device(const rkx_ode *p, int i, int j, double t, const double *y_dev,
                         double *f_dev, void *stream);
 
+/* The same on host buffers (y: all n components; writes f[i..j] only),
+ * staged through device memory: the host-side ode_eval_rng / ode_eval_comp
+ * callbacks (ode.h:34-35) of the drop-in problem plugins.  Synchronous. */
+int rkx_eval_rng_host(const rkx_ode *p, int i, int j, double t, const double *y, double *f);
+
 /* ---- methods (replaces rk_method_t, include/methods.h:92-118) ------------ */
 typedef struct rkx_method {
   char name[32];

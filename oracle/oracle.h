@@ -31,7 +31,7 @@
  *     reference counterpart; they are pinned by the invariants above
  *     (chain with zero diffusion == independent pairs; noise 0 == reference
  *     start vector) and by the convergence check against a tight-tolerance
- *     long-double run.
+ *     long-double run (orc_solve_ld, oracle_ld.c).
  */
 #ifndef RK_ORACLE_H_
 #define RK_ORACLE_H_
@@ -115,6 +115,10 @@ int orc_solve_PipeDls(const orc_ode *p, const orc_method *m, double t0, double t
 int orc_solve_pthreads_D(const orc_ode *p, const orc_method *m, double t0, double te,
                          const double *y0, double *y, double tol, const orc_opts *o,
                          int threads, orc_stats *st);
+/* accuracy reference: the D loop in long double at any tolerance (not a
+ * parity reference); y0 and y are double, the state is long double */
+int orc_solve_ld(const orc_ode *p, const char *method, double t0, double te, const double *y0,
+                 double *y, double tol, double h0, int step_limit, orc_stats *st);
 
 #ifdef __cplusplus
 }

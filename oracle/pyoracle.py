@@ -74,6 +74,8 @@ def lib():
         for fn in ("orc_solve_D", "orc_solve_PipeD", "orc_solve_PipeDls"):
             getattr(L, fn).argtypes = sig + [C.POINTER(Stats)]
         L.orc_solve_pthreads_D.argtypes = sig + [C.c_int, C.POINTER(Stats)]
+        L.orc_solve_ld.argtypes = [C.POINTER(Ode), C.c_char_p, C.c_double, C.c_double, dp, dp,
+                                   C.c_double, C.c_double, C.c_int, C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -171,6 +173,18 @@ def solve(p, meth, t0, te, y0, tol, h0=0.0, step_limit=0, impl="D", threads=1, t
     if tr is not None:
         tr = tr[:min(trace_cap, stats["steps"])]
     return y, stats, tr
+
+
+def solve_ld(p, meth, t0, te, y0, tol, h0=0.0, step_limit=0):
+    """The long-double accuracy reference (oracle_ld.c): (y rounded to double, stats)."""
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    y = np.empty_like(y0)
+    st = Stats()
+    rc = lib().orc_solve_ld(C.byref(p), meth.encode(), t0, te, _dp(y0), _dp(y), tol, h0,
+                            step_limit, C.byref(st))
+    if rc:
+        raise RuntimeError("long-double oracle solve failed: %d" % rc)
+    return y, dict(acc=st.steps_acc, rej=st.steps_rej, steps=st.steps_acc + st.steps_rej, t=st.t)
 
 
 def footprint(y):

@@ -29,6 +29,20 @@ SOURCES = [
     "rkx_kernels_chain.hip",
 ]
 
+# Reference-style solver binaries over the C ABI (rk_amd/csrc/dropin): name ->
+# (problem object, build-time defines as in the reference's Makefile)
+DROPIN = {
+    "rk_bruss2d_c1": ("bruss2d", "BRUSS_GRID_SIZE=128 T_END=4.0 TOL=1E-6 H_INIT=1E-3 "
+                                 "METHOD=DOPRI54 STEP_LIMIT=0"),
+    "rk_bruss2d_defaults": ("bruss2d", "BRUSS_GRID_SIZE=128"),
+    "rk_bruss2d_verner65_d": ("bruss2d", "BRUSS_GRID_SIZE=128 T_END=4.0 TOL=1E-8 H_INIT=1E-3 "
+                                         "METHOD=VERNER65 STEP_LIMIT=0 "
+                                         "RKX_DROPIN_IMPL=RKX_IMPL_D"),
+    "rk_lv_dopri54": ("lv", "LV_COUNT=10000 T_END=4.0 TOL=1E-6 H_INIT=1E-3 METHOD=DOPRI54 "
+                            "STEP_LIMIT=0"),
+}
+BIN = os.path.join(HERE, "bin")
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RKX_ARCH", "gfx950")
 
@@ -69,7 +83,18 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     shutil.move(tmp, LIB)
+    build_dropin()
     return LIB
+
+
+def build_dropin():
+    """The reference-style driver binaries of DROPIN, into rk_amd/bin/."""
+    for name, (problem, defines) in DROPIN.items():
+        cmd = ["make", "-s", "-C", os.path.join(CSRC, "dropin"), "PROBLEM=" + problem,
+               "DEFINES=" + defines, "OUT=" + os.path.join(BIN, name)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("drop-in build failed: %s\n%s" % (name, r.stdout + r.stderr))
 
 
 def main():

@@ -395,6 +395,27 @@ int rkx_eval_rng_device(const rkx_ode *p, int i, int j, double t, const double *
   return RKX_OK;
 }
 
+int rkx_eval_rng_host(const rkx_ode *p, int i, int j, double t, const double *y, double *f)
+{
+  if (int rc = check_ode(p))
+    return rc;
+  if (i < 0 || j >= p->n || j < i || !y || !f)
+    return fail(RKX_EINVAL, "rkx_eval_rng_host: bad range or pointer");
+  const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
+  const size_t cnt = static_cast<size_t>(j - i + 1) * sizeof(double);
+  solve_mem mem;
+  double *yd = nullptr, *fd = nullptr;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&yd), vec))
+    return rc;
+  if (int rc = mem.alloc(reinterpret_cast<void **>(&fd), vec))
+    return rc;
+  RKX_HIP(hipMemcpy(yd, y, vec, hipMemcpyHostToDevice));
+  if (int rc = rkx_eval_rng_device(p, i, j, t, yd, fd, nullptr))
+    return rc;
+  RKX_HIP(hipMemcpy(f + i, fd + i, cnt, hipMemcpyDeviceToHost));
+  return RKX_OK;
+}
+
 size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
 {
   if (check_ode(p) || check_method(m))

@@ -73,3 +73,19 @@ def test_bad_arguments_raise():
         rk_amd.method("NOPE")
     with pytest.raises(rk_amd.RkxError):
         rk_amd.LVChain(2)
+
+
+def test_dropin_binaries_built():
+    """The reference-style solver binaries exist, define solver() and the
+    ode.h / methods.h symbols, and load librkx.so from the package."""
+    import subprocess
+    from rk_amd.build import BIN, DROPIN
+    for name in DROPIN:
+        path = os.path.join(BIN, name)
+        assert os.path.exists(path), name
+        syms = subprocess.run(["nm", path], capture_output=True, text=True).stdout
+        for s in ("solver", "ode_start", "ode_eval_rng", "ode_eval_comp", "ode_acc_dist",
+                  "ode_size", "DOPRI54", "VERNER65"):
+            assert re.search(r" [TDB] %s$" % s, syms, re.M), (name, s)
+        ldd = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
+        assert "rk_amd/bin/../librkx.so" in ldd

@@ -1,0 +1,52 @@
+"""The reference-style binaries (solver() of solver.h:47 over the C ABI,
+built from rk_amd/csrc/dropin with the reference's build-time defines) print
+the reference's output contract, and their Result line equals the
+reference's known answer for the same defines, digit for digit (%.20e, so
+bitwise), with the same step counts.  Each binary runs as a child process."""
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(os.path.dirname(HERE), "rk_amd", "bin")
+KAT = {c["name"]: c for c in json.load(open(os.path.join(HERE, "golden", "kat.json")))["cases"]}
+
+CASES = [("rk_bruss2d_c1", "c1_bruss128_dopri54_tol1e-6"),
+         ("rk_bruss2d_defaults", "bruss128_defaults"),
+         ("rk_bruss2d_verner65_d", "bruss128_verner65_tol1e-8"),
+         ("rk_lv_dopri54", "lv10000_dopri54")]
+
+# the lines log2tab reads (log2tab:75-293), in its order
+F = r"\d+(\.\d+)?e[+-]\d+"
+LOG2TAB = [r"^ODE: \S+", r"^(Grid size|System size): \d+", r"^Integration interval: \[%s,%s\]" % (F, F),
+           r"^Initial value: ", r"^Tolerance: %s" % F, r"^Lock type: \S+", r"^Pad size: \d+",
+           r"^Solver type: \S+ embedded Runge-Kutta method", r"^Implementation variant: \S+",
+           r"^Method: \S+.*\(order \d+, \d+ stages\)", r"^Initial step size: \d+e[+-]\d+",
+           r"^Number of steps: \d+ \(\d+ accepted, \d+ rejected\)",
+           r"^Kernel time: %s s \(%s s per step\)" % (F, F), r"^Total time:\s+%s s" % F,
+           r"^Result: y\[0\] = %s, \.\.\., y\[\d+\] = %s;" % (F, F)]
+
+
+@pytest.mark.parametrize("binary,case", CASES)
+def test_dropin_binary_matches_reference(binary, case):
+    path = os.path.join(BIN, binary)
+    assert os.path.exists(path), "drop-in binaries are built by rk_amd/build.py"
+    print("running", binary, flush=True)
+    r = subprocess.run([path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    k = 0
+    for ln in lines:                      # every log2tab line, in order
+        if k < len(LOG2TAB) and re.match(LOG2TAB[k], ln):
+            k += 1
+    assert k == len(LOG2TAB), (LOG2TAB[k] if k < len(LOG2TAB) else "", r.stdout)
+    c = KAT[case]
+    m = re.search(r"^Number of steps: (\d+) \((\d+) accepted, (\d+) rejected\)", r.stdout, re.M)
+    assert (int(m.group(2)), int(m.group(3))) == (c["acc"], c["rej"])
+    m = re.search(r"^Result: y\[0\] = (\S+), \.\.\., y\[\d+\] = (\S+); min = (\S+), max = (\S+)$",
+                  r.stdout, re.M)
+    assert list(m.groups()) == c["footprint"]

@@ -226,6 +226,20 @@ def test_c2_full_interval(rk, case):
     assert list(got) == list(want)          # and in fact bitwise
 
 
+@pytest.mark.parametrize("meth,tol", [("DOPRI54", 1e-6), ("VERNER65", 1e-8)])
+def test_c1_accuracy_against_long_double(rk, orc, meth, tol):
+    """Config 1 on the device against the long-double reference at tol 1e-13
+    (the FP64 path's global error, not parity): within 10 tol."""
+    p = rk.Bruss2D(128)
+    y0 = p.ode_start()
+    po = orc.make_ode("bruss2d", 128)
+    yr, _ = orc.solve_ld(po, "DOPRI54", 0.0, 4.0, y0, 1e-13, h0=1e-3)
+    y, st, _ = run_gpu(rk, p, meth, 0.0, 4.0, y0, tol, h0=1e-3, impl="PipeDls")
+    err = max_rel(y, yr)
+    print("C1 %s tol %g: max rel error vs long double %.3e" % (meth, tol, err))
+    assert err <= 10 * tol
+
+
 def test_zero_length_interval(rk):
     p = rk.Bruss2D(8)
     y0 = p.ode_start()

@@ -189,3 +189,35 @@ def test_exponential_convergence(orc):
     ya, *_ = orc.solve(p, "DOPRI87", 0.0, 5.0, y0, 1e-13, h0=1e-4)
     yb, *_ = orc.solve(p, "DOPRI54", 0.0, 5.0, y0, 1e-10, h0=1e-4)
     assert np.abs(ya - yb).max() < 1e-8
+
+
+def test_long_double_reference_pinned(orc):
+    """The long-double accuracy reference (orc_solve_ld) is pinned by two
+    different tables agreeing at tight tolerances and by the LV pair's
+    conserved quantity V = d1 x - c1 ln x + d0 y - c0 ln y."""
+    p = orc.make_ode("lv", 1)
+    y0 = orc.ode_start(p)
+    ya, _ = orc.solve_ld(p, "DOPRI87", 0.0, 20.0, y0, 1e-16, h0=1e-4)
+    yb, _ = orc.solve_ld(p, "DOPRI54", 0.0, 20.0, y0, 1e-14, h0=1e-4)
+    assert np.abs(ya - yb).max() < 1e-11
+    c0, d0, d1, c1 = 0.1, 0.2, 0.4, 0.2
+    V = lambda x, y: d1 * x - c1 * np.log(x) + d0 * y - c0 * np.log(y)
+    assert abs(V(*ya) - V(*y0)) < 1e-11
+    pb = orc.make_ode("bruss2d", 16)
+    z0 = orc.ode_start(pb)
+    za, _ = orc.solve_ld(pb, "DOPRI54", 0.0, 4.0, z0, 1e-13, h0=1e-3)
+    zb, _ = orc.solve_ld(pb, "VERNER65", 0.0, 4.0, z0, 1e-13, h0=1e-3)
+    assert np.max(np.abs(za - zb) / np.abs(za)) < 5e-11
+
+
+@pytest.mark.parametrize("meth", ["DOPRI54", "VERNER65"])
+@pytest.mark.parametrize("tol", [1e-6, 1e-8])
+def test_fp64_accuracy_against_long_double(orc, meth, tol):
+    """Global error of the FP64 solve (the parity path) against the
+    long-double run at tol 1e-13: within 10 tol (relative, max over
+    components) on BRUSS2D N=16, t in [0, 4]."""
+    p = orc.make_ode("bruss2d", 16)
+    y0 = orc.ode_start(p)
+    yr, _ = orc.solve_ld(p, "DOPRI54", 0.0, 4.0, y0, 1e-13, h0=1e-3)
+    yd, _, _ = orc.solve(p, meth, 0.0, 4.0, y0, tol, h0=1e-3)
+    assert np.max(np.abs(yd - yr) / np.abs(yr)) <= 10 * tol
