@@ -10,6 +10,28 @@
 
 namespace rkx {
 
+// Vectors reached through an instance table are generic pointers to the
+// compiler, which then emits FLAT memory instructions; those count against
+// the LDS counter too, so every wait for LDS (before each barrier) would also
+// wait for the global loads in flight.  Kernels that prefetch through such
+// pointers view them in the global address space instead.
+using gconst = const __attribute__((address_space(1))) double;
+using gmut = __attribute__((address_space(1))) double;
+__device__ __forceinline__ gconst *as_global(const double *p) { return (gconst *) p; }
+__device__ __forceinline__ gmut *as_global(double *p) { return (gmut *) p; }
+
+// A value that is the same in every lane (a step coefficient), moved to
+// scalar registers: loaded through a generic pointer it would otherwise
+// occupy a vector register pair in every lane for the whole kernel.
+__device__ __forceinline__ double wave_uniform(double x)
+{
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
 // ---- BRUSS2D-MIX (src/problems/bruss2d-mix.c:66-206) ------------------------
 // Laplacian numerator of one component at grid point (k, j) given its
 // neighbours.  vN/vS already carry the Neumann mirror on the top/bottom rows

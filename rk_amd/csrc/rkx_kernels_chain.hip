@@ -67,13 +67,13 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
     static_for<s>([&](auto lt) {
       constexpr int l = decltype(lt)::value;
       if constexpr (l > i)
-        kv[i][l - i - 1] = nzA<P>(l, i) ? cf->col[i][l - i - 1] : 0.0;
+        kv[i][l - i - 1] = nzA<P>(l, i) ? wave_uniform(cf->col[i][l - i - 1]) : 0.0;
     });
     if constexpr (P::fsal && i < s - 1)
       kv[i][s - i - 1] = kv[i][s - i - 2];
     else
-      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? cf->col[i][s - i - 1] : 0.0;
-    kv[i][s - i] = ((P::e >> i) & 1u) ? cf->col[i][s - i] : 0.0;
+      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? wave_uniform(cf->col[i][s - i - 1]) : 0.0;
+    kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(cf->col[i][s - i]) : 0.0;
   });
 
   double2 yv[C];

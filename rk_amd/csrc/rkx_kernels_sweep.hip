@@ -51,7 +51,7 @@ __device__ __forceinline__ int reflect(int k, int N)
 }
 
 template <class P, int K>
-__global__ __launch_bounds__(128 * K) void k_sweep(const sweep_inst *__restrict__ insts,
+__global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ))) void k_sweep(const sweep_inst *__restrict__ insts,
                                                    const int2 *__restrict__ tiles, int ntiles)
 {
   constexpr int s = P::s;
@@ -78,8 +78,8 @@ __global__ __launch_bounds__(128 * K) void k_sweep(const sweep_inst *__restrict_
   if (st->done | st->pow_wait)
     return;                                      // uniform over the instance's tiles
   const int cur = st->cur;
-  const double *__restrict__ y = cur ? I.y1 : I.y0;
-  double *__restrict__ yn = cur ? I.y0 : I.y1;
+  gconst *__restrict__ y = as_global(cur ? I.y1 : I.y0);
+  gmut *__restrict__ yn = as_global(cur ? I.y0 : I.y1);
   const int N = I.N;
   const int N2 = N + N;
   const double coef = I.coef;
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(128 * K) void k_sweep(const sweep_inst *__restrict_
   double yr[NS];                                 // y rows of stage 0, ring by row
   double yo[NS];                                 // y_old of finishing rows, ring by iteration
   constexpr int DY = 2;                          // y_old loaded DY iterations ahead
-  const double *yc = y + 2 * ccol + var;
+  gconst *yc = y + 2 * ccol + var;
   auto yrow = [&](int k) -> double { return yc[static_cast<size_t>(reflect(k, N)) * N2]; };
   static_for<PF + 2>([&](auto dt) {
     constexpr int d = decltype(dt)::value - 1;   // rows it0-1 .. it0+PF
@@ -141,13 +141,13 @@ __global__ __launch_bounds__(128 * K) void k_sweep(const sweep_inst *__restrict_
     static_for<s>([&](auto lt) {
       constexpr int l = decltype(lt)::value;
       if constexpr (l > i)
-        kv[i][l - i - 1] = nzA<P>(l, i) ? cf->col[i][l - i - 1] : 0.0;
+        kv[i][l - i - 1] = nzA<P>(l, i) ? wave_uniform(cf->col[i][l - i - 1]) : 0.0;
     });
     if constexpr (P::fsal && i < s - 1)
       kv[i][s - i - 1] = kv[i][s - i - 2];
     else
-      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? cf->col[i][s - i - 1] : 0.0;
-    kv[i][s - i] = ((P::e >> i) & 1u) ? cf->col[i][s - i] : 0.0;
+      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? wave_uniform(cf->col[i][s - i - 1]) : 0.0;
+    kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(cf->col[i][s - i]) : 0.0;
   });
 
   double qmax = 0.0;

@@ -9,38 +9,45 @@
 
 namespace rkx {
 
-// Zero patterns of the compiled methods (bit i of row l: A[l][i] != 0).
+// Zero patterns of the compiled methods (bit i of row l: A[l][i] != 0), and
+// the occupancy (waves per SIMD) the fused kernels are register-allocated for.
 struct pat_dopri54 {
+  static constexpr int occ = 2;
   static constexpr int s = 7;
   static constexpr unsigned rowA[7] = {0x0, 0x1, 0x3, 0x7, 0xF, 0x1F, 0x3D};
   static constexpr unsigned b = 0x3D, e = 0x7D;
   static constexpr bool fsal = true;      // A[6][j] == b[j]
 };
 struct pat_verner65 {
+  static constexpr int occ = 1;
   static constexpr int s = 8;
   static constexpr unsigned rowA[8] = {0x0, 0x1, 0x3, 0x7, 0xF, 0x1F, 0x1F, 0x5F};
   static constexpr unsigned b = 0x3D, e = 0xFD;
   static constexpr bool fsal = false;
 };
 struct pat_rkf23 {
+  static constexpr int occ = 2;
   static constexpr int s = 3;
   static constexpr unsigned rowA[3] = {0x0, 0x1, 0x3};
   static constexpr unsigned b = 0x3, e = 0x7;
   static constexpr bool fsal = false;
 };
 struct pat_heun2 {
+  static constexpr int occ = 2;
   static constexpr int s = 2;
   static constexpr unsigned rowA[2] = {0x0, 0x1};
   static constexpr unsigned b = 0x3, e = 0x3;
   static constexpr bool fsal = false;
 };
 struct pat_rk4 {
+  static constexpr int occ = 2;
   static constexpr int s = 4;
   static constexpr unsigned rowA[4] = {0x0, 0x1, 0x2, 0x4};
   static constexpr unsigned b = 0xF, e = 0xF;
   static constexpr bool fsal = false;
 };
 struct pat_ssprk3 {
+  static constexpr int occ = 2;
   static constexpr int s = 3;
   static constexpr unsigned rowA[3] = {0x0, 0x1, 0x3};
   static constexpr unsigned b = 0x7, e = 0x7;
