@@ -16,6 +16,10 @@ cpu_baseline: the reference's pthreads PipeD4 (built from the reference's C
 sources into oracle/_ref) on the host's cores for a 200-step sample of the
 same workload, or the oracle's pthreads D restatement when that binary is
 absent.
+batched / ensemble / serving: the batched solves/s against plain per-instance
+launches on the 512-instance mixed BRUSS2D batch, the LV ensembles, and the
+continuous-batching server's p50/p99 latency and solves/s against the plain
+solver on a Poisson request stream (see the leg functions for the workloads).
 """
 import argparse
 import json
@@ -70,6 +74,119 @@ def cpu_baseline(threads):
                       "attempted steps"}
 
 
+def batched_leg(sample_every=2):
+    """Mixed batch of 512 BRUSS2D instances (N uniform on 32..512 from seed 0,
+    alpha log-uniform on [1e-3, 5e-3], noisy u0; DOPRI54 tol 1e-6, t in
+    [0, 4], h0 1e-3): all of them through the batch engine vs the plain
+    solver launched per instance (timed on every sample_every-th instance and
+    scaled by sample_every)."""
+    import torch
+    import rk_amd
+    from rk_amd.workloads import bruss_instance, mixed_bruss
+    inst = [bruss_instance(N, a, k) for k, (N, a) in enumerate(mixed_bruss(512, seed=0))]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    with rk_amd.Batch() as b:
+        ids = [b.add(p, "DOPRI54", 0.0, 4.0, y0, 1e-6, h0=1e-3) for p, y0 in inst]
+        left = b.run(10 ** 9)
+        t_b = time.perf_counter() - t
+        sts = [b.status(i)[0] for i in ids]
+    assert left == 0
+    steps = sum(s.steps for s in sts)
+    comp_steps = sum(p.n * s.steps for (p, _), s in zip(inst, sts))
+    t_p = 0.0
+    for k in range(0, len(inst), sample_every):
+        p, y0 = inst[k]
+        y0d = torch.from_numpy(y0).cuda()
+        yd = torch.empty_like(y0d)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rk_amd.solve(p, "DOPRI54", 0.0, 4.0, y0d, yd, 1e-6, h0=1e-3)
+        torch.cuda.synchronize()
+        t_p += time.perf_counter() - t
+    t_p *= sample_every
+    return {"workload": "512 BRUSS2D instances, N uniform 32..512 (seed 0), alpha log-uniform "
+                        "[1e-3,5e-3], DOPRI54 tol 1e-6, t in [0,4]",
+            "batched_s": t_b, "batched_solves_per_s": len(inst) / t_b,
+            "plain_s": t_p, "plain_solves_per_s": len(inst) / t_p,
+            "speedup_vs_plain": t_p / t_b, "attempted_steps": steps,
+            "hbm_gbs_algorithmic": 16.0 * comp_steps / t_b / 1e9,
+            "plain_sample": "plain solver timed on every %d-th instance (%d of %d), scaled by %d"
+                            % (sample_every, len(range(0, len(inst), sample_every)), len(inst),
+                               sample_every)}
+
+
+def ensemble_leg():
+    """LV batch: (A) 262144 independent pairs, VERNER65 tol 1e-8, t in [0, 20];
+    (B) 64 periodic diffusive chains, M uniform on 1e4..1e6 (seed 1), DOPRI54
+    tol 1e-6, t in [0, 4], through the batch engine vs plain per chain."""
+    import torch
+    import rk_amd
+    from rk_amd.ensemble import lv_ensemble, lv_ensemble_inputs
+    from rk_amd.workloads import chain_instance, lv_chains
+    params, y0 = lv_ensemble_inputs(262144, seed=1)
+    P = torch.from_numpy(params).cuda()
+    Y0 = torch.from_numpy(y0).cuda()
+    lv_ensemble(P, Y0, 0.0, 20.0, "VERNER65", 1e-8)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, acc, rej = lv_ensemble(P, Y0, 0.0, 20.0, "VERNER65", 1e-8)
+    torch.cuda.synchronize()
+    t_a = time.perf_counter() - t
+    att = (acc + rej).double()
+    chains = [chain_instance(M, k) for k, M in enumerate(lv_chains(64, seed=1))]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    with rk_amd.Batch() as b:
+        for p, c0 in chains:
+            b.add(p, "DOPRI54", 0.0, 4.0, c0, 1e-6, h0=1e-3)
+        b.run(10 ** 9)
+    t_b = time.perf_counter() - t
+    t_p = 0.0
+    for p, c0 in chains:
+        y0d = torch.from_numpy(c0).cuda()
+        yd = torch.empty_like(y0d)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rk_amd.solve(p, "DOPRI54", 0.0, 4.0, y0d, yd, 1e-6, h0=1e-3)
+        torch.cuda.synchronize()
+        t_p += time.perf_counter() - t
+    return {"pairs": {"systems": 262144, "seconds": t_a, "solves_per_s": 262144 / t_a,
+                      "mean_attempted_steps": float(att.mean()),
+                      "max_attempted_steps": int(att.max())},
+            "chains": {"instances": len(chains), "batched_s": t_b, "plain_s": t_p,
+                       "batched_solves_per_s": len(chains) / t_b,
+                       "plain_solves_per_s": len(chains) / t_p}}
+
+
+def serving_leg(rate_hz=4.0, duration_s=60.0, hbm_budget=4 << 30):
+    """Poisson stream (seed 2) at rate_hz over duration_s: 70 % BRUSS2D N
+    uniform 64..1024, 30 % LV chains M uniform 1e4..5e5, tol from {1e-4,
+    1e-6, 1e-8}, t_end uniform [0.5, 4], DOPRI54, h0 1e-3.  Served in real
+    time by the continuous-batching server under an HBM budget, and by the
+    plain solver one request at a time (each solve timed, queueing from the
+    arrival times)."""
+    from rk_amd.problems import Bruss2D, LVChain
+    from rk_amd.serve import Request, serve_batched, serve_plain
+    from rk_amd.workloads import serving_stream
+    stream = serving_stream(rate_hz, duration_s, seed=2)
+
+    def requests():
+        out = []
+        for k, (t, kind, size, tol, te) in enumerate(stream):
+            p = Bruss2D(size, 2e-3) if kind == "bruss2d" else LVChain(size, diff=1e-3)
+            y0 = p.ode_start_seeded(2, k, 0.01 if kind == "bruss2d" else 0.0)
+            out.append(Request(k, t, p, "DOPRI54", 0.0, te, y0, tol, h0=1e-3, kind=kind))
+        return out
+    _, sb = serve_batched(requests(), hbm_budget=hbm_budget)
+    _, sp = serve_plain(requests())
+    return {"workload": "Poisson %.1f req/s over %.0f s (seed 2): 70%% BRUSS2D N 64..1024, "
+                        "30%% LV chain M 1e4..5e5, tol {1e-4,1e-6,1e-8}, t_end U[0.5,4]"
+                        % (rate_hz, duration_s),
+            "hbm_budget_bytes": hbm_budget, "batched": sb, "plain": sp,
+            "p99_latency_ratio_plain_over_batched": sp["p99_latency_s"] / sb["p99_latency_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +198,10 @@ def main():
     ap.add_argument("--tile-waves", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched, ensemble and serving legs")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--step-limit", type=int, default=0,
                     help="cap each solve at this many attempted steps (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -167,6 +288,14 @@ def main():
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)
+    if not a.no_batch and a.step_limit == 0:
+        res["batched"] = batched_leg()
+        print("batched: %.2f s vs plain %.2f s" % (res["batched"]["batched_s"],
+                                                   res["batched"]["plain_s"]),
+              file=sys.stderr, flush=True)
+        res["ensemble"] = ensemble_leg()
+        print("ensemble done", file=sys.stderr, flush=True)
+        res["serving"] = serving_leg(a.serve_rate, a.serve_seconds)
     print(json.dumps(res))
     return 0
 

@@ -78,6 +78,10 @@ def serve_batched(requests, hbm_budget=64 << 30, poll_steps=64, realtime=True, c
                 r = pending[nxt]
                 i = b.add(r.problem, r.method, r.t0, r.te, r.y0, r.tol, h0=r.h0)
                 if i is None:
+                    if not active:
+                        raise RuntimeError("request %d needs %d bytes, more than the HBM "
+                                           "budget of %d" % (r.rid, Batch.footprint(
+                                               r.problem, r.method), hbm_budget))
                     break                      # budget full: wait for retirements
                 r.admitted = now()
                 active[i] = r

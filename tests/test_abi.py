@@ -89,3 +89,17 @@ def test_dropin_binaries_built():
             assert re.search(r" [TDB] %s$" % s, syms, re.M), (name, s)
         ldd = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
         assert "rk_amd/bin/../librkx.so" in ldd
+
+
+def test_serving_summary_statistics():
+    """Latency percentiles and throughput of the serving summary (host logic)."""
+    from rk_amd.serve import Request, summarize
+    rs = []
+    for k in range(100):
+        r = Request(k, float(k), None, "DOPRI54", 0.0, 1.0, np.zeros(2), 1e-6)
+        r.finished = r.arrival + (k + 1) * 0.01
+        rs.append(r)
+    s = summarize(rs, 200.0)
+    assert s["requests"] == 100 and s["solves_per_s"] == 0.5
+    assert abs(s["p50_latency_s"] - 0.505) < 1e-12
+    assert abs(s["p99_latency_s"] - np.percentile(np.arange(1, 101) * 0.01, 99)) < 1e-12

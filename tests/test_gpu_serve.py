@@ -1,0 +1,67 @@
+"""The continuous-batching server: requests that arrive while others run are
+admitted into the running batch (HBM budget permitting), and every request's
+result equals the plain single-instance solve of the same request bitwise
+(same device step control), whatever the mix it ran in."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rk():
+    import torch
+    assert torch.cuda.is_available()
+    import rk_amd
+    rk_amd.lib()
+    return rk_amd
+
+
+def stream(rk):
+    from rk_amd.serve import Request
+    items = [(0.00, "bruss2d", 40, 1e-6, 0.6), (0.01, "lvchain", 5000, 1e-8, 2.0),
+             (0.02, "bruss2d", 97, 1e-4, 0.5), (0.05, "bruss2d", 24, 1e-8, 1.0),
+             (0.08, "lvchain", 12000, 1e-6, 1.5), (0.10, "bruss2d", 130, 1e-6, 0.4),
+             (0.12, "bruss2d", 64, 1e-6, 0.8), (0.15, "lvchain", 3001, 1e-4, 3.0)]
+    out = []
+    for k, (t, kind, size, tol, te) in enumerate(items):
+        p = rk.Bruss2D(size, 2e-3) if kind == "bruss2d" else rk.LVChain(size, diff=1e-3)
+        y0 = p.ode_start_seeded(2, k, 0.01 if kind == "bruss2d" else 0.0)
+        out.append(Request(k, t, p, "DOPRI54", 0.0, te, y0, tol, h0=1e-3, kind=kind))
+    return out
+
+
+def plain_results(rk, reqs):
+    from rk_amd.serve import serve_plain
+    done, _ = serve_plain(reqs, keep_results=True)
+    return {r.rid: (r.result, r.steps) for r in done}
+
+
+@pytest.mark.parametrize("realtime,budget", [(True, 64 << 30), (False, 64 << 30),
+                                             (False, 800_000)])
+def test_served_results_equal_plain(rk, realtime, budget):
+    from rk_amd.serve import serve_batched
+    want = plain_results(rk, stream(rk))
+    done, summ = serve_batched(stream(rk), hbm_budget=budget, poll_steps=16,
+                               realtime=realtime, keep_results=True)
+    assert summ["requests"] == len(want)
+    for r in done:
+        y, steps = want[r.rid]
+        assert r.steps == steps, r.rid
+        assert np.array_equal(r.result, y), r.rid
+        assert r.admitted >= (r.arrival if realtime else 0.0) and r.finished >= r.admitted
+
+
+def test_budget_limits_resident_set(rk):
+    """An 800 kB budget holds only a few of the requests at a time (about 1.7
+    MB in all): the server queues the rest and admits them as earlier ones
+    retire."""
+    from rk_amd.batch import Batch
+    reqs = stream(rk)
+    need = [Batch.footprint(r.problem, r.method) for r in reqs]
+    assert sum(need) > 800_000 and max(need) <= 800_000
+    from rk_amd.serve import serve_batched
+    done, _ = serve_batched(reqs, hbm_budget=800_000, realtime=False)
+    assert len(done) == len(reqs)
+    # some request had to wait for a retirement
+    assert max(r.admitted for r in done) > min(r.finished for r in done)
