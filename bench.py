@@ -83,10 +83,15 @@ def batched_leg(sample_every=2):
     import torch
     import rk_amd
     from rk_amd.workloads import bruss_instance, mixed_bruss
+    from rk_amd.autotune import batch_tiles
     inst = [bruss_instance(N, a, k) for k, (N, a) in enumerate(mixed_bruss(512, seed=0))]
     torch.cuda.synchronize()
     t = time.perf_counter()
-    with rk_amd.Batch() as b:
+    (tw, tr), cand = batch_tiles(inst)
+    t_tune = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    with rk_amd.Batch(tile_waves=tw, tile_rows=tr) as b:
         ids = [b.add(p, "DOPRI54", 0.0, 4.0, y0, 1e-6, h0=1e-3) for p, y0 in inst]
         left = b.run(10 ** 9)
         t_b = time.perf_counter() - t
@@ -110,6 +115,8 @@ def batched_leg(sample_every=2):
             "batched_s": t_b, "batched_solves_per_s": len(inst) / t_b,
             "plain_s": t_p, "plain_solves_per_s": len(inst) / t_p,
             "speedup_vs_plain": t_p / t_b, "attempted_steps": steps,
+            "tiles": {"waves": tw, "rows": tr, "autotune_s": t_tune,
+                      "candidates_s": {"%dx%d" % k: v for k, v in cand.items()}},
             "hbm_gbs_algorithmic": 16.0 * comp_steps / t_b / 1e9,
             "plain_sample": "plain solver timed on every %d-th instance (%d of %d), scaled by %d"
                             % (sample_every, len(range(0, len(inst), sample_every)), len(inst),

@@ -161,7 +161,8 @@ enum {
 typedef struct rkx_batch_opts {
   size_t hbm_budget;      /* device bytes the batch may hold for instance data (0: no limit) */
   int ctrl;               /* RKX_CTRL_DEVICE or RKX_CTRL_DEVICE_POW */
-  int tile_waves;         /* fused-kernel strip width for all instances (0: chosen per size) */
+  int tile_waves;         /* fused-kernel strip width for all instances (0: chosen per size;
+                             rk_amd/autotune.py picks both by timing the batch's own work) */
   int tile_rows;          /* fused-kernel band height (0: chosen per size) */
   void *stream;           /* hipStream_t, NULL: the library's own stream */
 } rkx_batch_opts;

@@ -1,0 +1,44 @@
+"""Tile-shape autotuner for batches of fused BRUSS2D instances.
+
+The fused kernel's strip width (1..4 wavefronts per component: 64..256 grid
+columns per workgroup) and band height (rows per workgroup) trade wasted
+lanes and halo rows against parallelism and occupancy, and the best trade
+depends on the mix of grid sizes and on how many workgroups of other
+instances share the CUs -- which a per-kernel cost model does not capture.
+batch_tiles() therefore measures: it runs the same batched steps of the
+batch's largest instances (the ones that dominate the run time: a grid's
+work per step grows as N^2 and its step count with N^2 too, for explicit
+diffusion) under each candidate shape and returns the fastest.  Every
+candidate performs identical arithmetic (the tile shape never changes
+results), so wall times compare directly.
+"""
+import time
+
+CANDIDATES = ((1, 64), (2, 48), (2, 64), (2, 96), (2, 128), (4, 64), (4, 96))
+
+
+def batch_tiles(instances, meth="DOPRI54", t0=0.0, te=4.0, tol=1e-6, h0=1e-3, sample=48,
+                steps=48, candidates=CANDIDATES):
+    """instances: [(problem, y0), ...] of BRUSS2D problems.  Returns
+    ((tile_waves, tile_rows), {(K, R): seconds}) for the fastest candidate."""
+    import torch
+    from .batch import Batch
+    from .methods import method
+    s = method(meth).s
+    order = sorted(range(len(instances)), key=lambda k: -instances[k][0].n)
+    pick = [instances[k] for k in order[:sample]]
+    times = {}
+    for K, R in candidates:
+        if 64 * K <= 2 * s:
+            continue
+        with Batch(tile_waves=K, tile_rows=R) as b:
+            for p, y0 in pick:
+                b.add(p, meth, t0, te, y0, tol, h0=h0)
+            b.run(16)                          # tables, graph capture, warm caches
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            b.run(steps)
+            torch.cuda.synchronize()
+            times[(K, R)] = time.perf_counter() - t
+    best = min(times, key=times.get)
+    return best, times

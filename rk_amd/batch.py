@@ -22,6 +22,10 @@ class BatchOpts(C.Structure):
 
 
 class Batch:
+    """tile_waves / tile_rows: the fused kernel's strip width (wavefronts per
+    component) and band height for every instance (0: chosen per size;
+    rk_amd.autotune.batch_tiles picks them by timing the batch's own work)."""
+
     def __init__(self, hbm_budget=0, ctrl="device", tile_waves=0, tile_rows=0, stream=None):
         o = BatchOpts()
         lib().rkx_batch_opts_default(C.byref(o))

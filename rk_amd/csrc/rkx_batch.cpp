@@ -62,7 +62,11 @@ struct rkx_batch {
   dev_method *dm_dev = nullptr;
   sweep_inst *si_dev = nullptr;
   chain_inst *ci_dev = nullptr;
-  state *st_pinned = nullptr;
+  state *st_pinned = nullptr;          // 2 x kBatchCap: snapshots of two chunks in flight
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;           // kPollSteps batched steps, captured once per table set
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_ok = false;
   std::vector<batch_inst> inst;
   std::map<std::pair<int, int>, tile_group> groups;
   size_t bytes = 0;
@@ -81,16 +85,15 @@ void free_inst(batch_inst &b)
   b = batch_inst{};
 }
 
-// strip width and band height of a batched BRUSS2D instance: narrow strips
-// for small grids, and bands of about 64 rows (long enough for the s-1 halo
-// rows to be a small share, short enough that large instances split into
-// many tiles)
+// Strip width and band height of a batched BRUSS2D instance: one wavefront
+// per component for grids that fit a 64-column strip, else two (measured
+// fastest per lane when many instances share the CUs), and bands of 64 rows
+// (long enough for the s-1 halo rows to be a small share, short enough that
+// large instances split into many tiles).  The options override either.
 sweep_cfg batch_cfg(const dev_ode &p, const rkx_method &m, const rkx_batch_opts &o)
 {
   sweep_cfg c;
-  c.waves = (p.N <= 64) ? 1 : 2;
-  if (64 * c.waves <= 2 * m.s)
-    c.waves = 2;
+  c.waves = (p.N + 2 * m.s <= 64) ? 1 : 2;
   c.rows = 64;
   if (o.tile_waves > 0)
     c.waves = o.tile_waves;
@@ -191,6 +194,89 @@ int poll_states(rkx_batch *b, int *newly_done)
   return RKX_OK;
 }
 
+void drop_graph(rkx_batch *b)
+{
+  if (b->graph_exec)
+    (void) hipGraphExecDestroy(b->graph_exec);
+  if (b->graph)
+    (void) hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  b->graph_ok = false;
+}
+
+// Capture kPollSteps batched steps of the current tables as one graph.  The
+// capture is always ended, also when a launch fails inside it.
+int capture_chunk(rkx_batch *b)
+{
+  drop_graph(b);
+  RKX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = RKX_OK;
+  for (int j = 0; j < kPollSteps && rc == RKX_OK; ++j)
+    rc = enqueue_batched_step(b);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc != RKX_OK) {
+    if (g)
+      (void) hipGraphDestroy(g);
+    return rc;
+  }
+  RKX_HIP(e);
+  b->graph = g;
+  RKX_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+  b->graph_ok = true;
+  return RKX_OK;
+}
+
+// Enqueue k batched steps and a snapshot of every instance's state into
+// snapshot buffer `slot`; full chunks replay the captured graph.
+int launch_chunk(rkx_batch *b, int slot, int k)
+{
+  if (b->dirty) {
+    if (int rc = rebuild_tables(b))
+      return rc;
+    b->graph_ok = false;
+  }
+  if (k == kPollSteps) {
+    if (!b->graph_ok)
+      if (int rc = capture_chunk(b))
+        return rc;
+    RKX_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  } else {
+    for (int j = 0; j < k; ++j)
+      if (int rc = enqueue_batched_step(b))
+        return rc;
+  }
+  RKX_HIP(hipMemcpyAsync(b->st_pinned + static_cast<size_t>(slot) * kBatchCap, b->st_dev,
+                         sizeof(state) * b->hi_slot, hipMemcpyDeviceToHost, b->stream));
+  RKX_HIP(hipEventRecord(b->ev[slot], b->stream));
+  return RKX_OK;
+}
+
+// Read the snapshot of `slot` once its chunk has finished: update the host
+// copies, count newly finished instances, report pending pow time-outs.
+int read_snapshot(rkx_batch *b, int slot, int *newly_done, bool *pow_wait)
+{
+  RKX_HIP(hipEventSynchronize(b->ev[slot]));
+  const state *snap = b->st_pinned + static_cast<size_t>(slot) * kBatchCap;
+  for (int i = 0; i < b->hi_slot; ++i) {
+    batch_inst &I = b->inst[static_cast<size_t>(i)];
+    if (!I.used)
+      continue;
+    const state &s = snap[i];
+    if (s.pow_wait) {
+      *pow_wait = true;
+      continue;
+    }
+    I.snap = s;
+    if (s.done && !I.done) {
+      I.done = true;
+      ++*newly_done;
+    }
+  }
+  return RKX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -233,7 +319,12 @@ int rkx_batch_create(rkx_batch **out, const rkx_batch_opts *o)
   dalloc(reinterpret_cast<void **>(&b->dm_dev), sizeof(dev_method) * kBatchCap);
   dalloc(reinterpret_cast<void **>(&b->si_dev), sizeof(sweep_inst) * kBatchCap);
   dalloc(reinterpret_cast<void **>(&b->ci_dev), sizeof(chain_inst) * kBatchCap);
-  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&b->st_pinned), sizeof(state) * kBatchCap,
+  for (int k = 0; k < 2 && !rc; ++k)
+    if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) {
+      (void) hipGetLastError();
+      rc = fail(RKX_EDEVICE, "rkx_batch_create: event creation failed");
+    }
+  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&b->st_pinned), 2 * sizeof(state) * kBatchCap,
                            hipHostMallocDefault) != hipSuccess) {
     (void) hipGetLastError();
     rc = fail(RKX_ENOMEM, "rkx_batch_create: pinned allocation failed");
@@ -269,6 +360,10 @@ int rkx_batch_destroy(rkx_batch *b)
       (void) hipFree(p);
   if (b->st_pinned)
     (void) hipHostFree(b->st_pinned);
+  drop_graph(b);
+  for (hipEvent_t e : b->ev)
+    if (e)
+      (void) hipEventDestroy(e);
   delete b;
   return RKX_OK;
 }
@@ -403,22 +498,61 @@ int rkx_batch_run(rkx_batch *b, int max_steps, int *active)
       a += (b->inst[static_cast<size_t>(i)].used && !b->inst[static_cast<size_t>(i)].done);
     return a;
   };
-  int done_steps = 0;
-  while (done_steps < max_steps && count_active() > 0) {
-    if (b->dirty)
-      if (int rc = rebuild_tables(b))
-        return rc;
-    const int k = std::min(kPollSteps, max_steps - done_steps);
-    for (int j = 0; j < k; ++j)
-      if (int rc = enqueue_batched_step(b))
-        return rc;
-    done_steps += k;
-    int newly = 0;
-    if (int rc = poll_states(b, &newly))
+  // Chunks of kPollSteps batched steps (a captured graph), two in flight:
+  // chunk c+1 and its state snapshot are enqueued before the host reads the
+  // snapshot of chunk c.  Within a run the active set only shrinks, and the
+  // tiles of a finished instance return at once, so a chunk enqueued before
+  // the host saw an instance finish is still correct; the tables (and the
+  // graph) are rebuilt before the next chunk after one does.
+  int enqueued = 0;
+  bool pending[2] = {false, false};
+  int cur = 0;
+  if (max_steps > 0 && count_active() > 0) {
+    const int k = std::min(kPollSteps, max_steps);
+    if (int rc = launch_chunk(b, cur, k))
       return rc;
+    enqueued += k;
+    pending[cur] = true;
+  }
+  while (pending[cur]) {
+    const int nxt = cur ^ 1;
+    if (enqueued < max_steps && count_active() > 0) {
+      const int k = std::min(kPollSteps, max_steps - enqueued);
+      if (int rc = launch_chunk(b, nxt, k))
+        return rc;
+      enqueued += k;
+      pending[nxt] = true;
+    }
+    int newly = 0;
+    bool pw = false;
+    if (int rc = read_snapshot(b, cur, &newly, &pw))
+      return rc;
+    pending[cur] = false;
+    if (pw) {
+      // a pow request of some instance timed out: drain the stream (that
+      // instance's kernels return at once meanwhile), finish its step
+      // control on the host, and continue from the device's current states
+      RKX_HIP(hipStreamSynchronize(b->stream));
+      pending[nxt] = false;
+      int more = 0;
+      if (int rc = poll_states(b, &more))
+        return rc;
+      newly += more;
+      if (enqueued < max_steps && count_active() > 0) {
+        const int k = std::min(kPollSteps, max_steps - enqueued);
+        if (newly)
+          b->dirty = true;
+        if (int rc = launch_chunk(b, nxt, k))
+          return rc;
+        enqueued += k;
+        pending[nxt] = true;
+      }
+    }
     if (newly)
       b->dirty = true;
+    cur = nxt;
   }
+  RKX_HIP(hipStreamSynchronize(b->stream));
   if (active)
     *active = count_active();
   return RKX_OK;

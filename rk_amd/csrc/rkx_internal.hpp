@@ -210,6 +210,11 @@ struct sweep_inst {
 bool sweep_supported(const dev_ode &p, const rkx_method &m);
 int sweep_pattern(const rkx_method &m);   // compiled zero pattern (0: none)
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m);
+// Tile-shape cost model: relative cost of one lane x row iteration for strips
+// of K = 1..4 wavefronts per component (f[K-1]); f == nullptr or f[K-1] <= 0
+// selects the measured defaults.  Returns the modelled cost of one step of an
+// N x N instance with strips of 64K columns and bands of R rows.
+double sweep_tile_cost(int N, int s, int K, int R, const double *f);
 sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
                            const dev_coef *cf, double *y0, double *y1);
 // tiles == nullptr: a single instance (insts[0]) with ntiles tiles

@@ -323,6 +323,27 @@ bool sweep_supported(const dev_ode &p, const rkx_method &m)
   return p.kind == RKX_BRUSS2D && p.N >= 2 && pattern_of(m) != 0;
 }
 
+// Relative cost of one lane x row iteration of k_sweep for strips of 1..4
+// waves per component, measured on MI355X (DOPRI54, N = 2000, one dispatch
+// round): 1 wave pays a barrier and an LDS exchange for few lanes, 3 waves
+// (6 per workgroup) do not pack two workgroups per CU.
+constexpr double kTileCost[4] = {1.36, 1.0, 1.67, 0.97};
+
+double sweep_tile_cost(int N, int s, int K, int R, const double *f)
+{
+  const int L = 64 * K;
+  const int oc = L - 2 * s;
+  if (oc < 1 || R < 1 || N < 1)
+    return 1e300;
+  const double fk = (f && f[K - 1] > 0.0) ? f[K - 1] : kTileCost[K - 1];
+  const long long nstrips = (N + oc - 1) / oc;
+  const long long nbands = (N + R - 1) / R;
+  const long long reff = (N + nbands - 1) / nbands;
+  // + 4: the prologue of a tile (coefficients, first rows) in row iterations
+  return fk * static_cast<double>(nstrips * L) * static_cast<double>(nbands) *
+         static_cast<double>(reff + 2 * (s - 1) + 4);
+}
+
 // Default tile: strips of 64*K columns (2K waves per workgroup) and the band
 // height R for which all strips x bands tiles are resident at once (one
 // dispatch round: a second, partial round would leave most CUs idle while
@@ -330,13 +351,8 @@ bool sweep_supported(const dev_ode &p, const rkx_method &m)
 // recomputes are a smaller share of taller bands).
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
 {
-  sweep_cfg c;
   const int s = m.s;
   const int N = p.N;
-  c.waves = (N <= 64) ? 1 : 2;
-  const int L = 64 * c.waves;
-  const int oc = L - 2 * s;
-  const int nstrips = (N > 0) ? (N + oc - 1) / oc : 1;
   int cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -344,14 +360,29 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     (void) hipGetLastError();
     cus = 256;
   }
-  const int slots = cus * resident_blocks(pattern_of(m), c.waves);
-  int bands = slots / nstrips;
-  if (bands < 1)
-    bands = 1;
-  c.rows = (N + bands - 1) / bands;
-  if (c.rows < 4)
-    c.rows = 4;
-  return c;
+  // for each strip width, the band height that fills one dispatch round;
+  // then the width with the lowest modelled cost
+  sweep_cfg best{2, 4};
+  double best_cost = 1e300;
+  for (int K = 1; K <= 4; ++K) {
+    const int oc = 64 * K - 2 * s;
+    if (oc < 1)
+      continue;
+    const int nstrips = (N + oc - 1) / oc;
+    const int slots = cus * resident_blocks(pattern_of(m), K);
+    int bands = slots / nstrips;
+    if (bands < 1)
+      bands = 1;
+    int rows = (N + bands - 1) / bands;
+    if (rows < 4)
+      rows = 4;
+    const double cost = sweep_tile_cost(N, s, K, rows, nullptr);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = sweep_cfg{K, rows};
+    }
+  }
+  return best;
 }
 
 // Geometry of one instance's tiles for strips of 64*K columns and bands of R rows.

@@ -98,7 +98,8 @@ def test_lv_rhs_bitwise(rk, orc, kind):
 
 # ---- full solves, bitwise with host step control ------------------------------
 
-CFGS = [("D", 0, 0), ("PipeDls", 1, 16), ("PipeDls", 2, 7), ("PipeDls", 4, 32), ("PipeD", 4, 100)]
+CFGS = [("D", 0, 0), ("PipeDls", 1, 16), ("PipeDls", 2, 7), ("PipeDls", 3, 20),
+        ("PipeDls", 4, 32), ("PipeD", 4, 100)]
 
 
 @pytest.mark.parametrize("impl,waves,rows", CFGS)
