@@ -74,6 +74,43 @@ def cpu_baseline(threads):
                       "attempted steps"}
 
 
+def d_form_passes(s):
+    """Vector passes (reads + writes of n doubles) of one step of the stage
+    kernels of the D form: first stage reads y and writes w_1..w_{s-1}, dy,
+    err; intermediate stage i reads w_i and updates w_{i+1..s-1}, dy, err;
+    the last reads w_{s-1}, dy, err, y and writes y_new."""
+    return (s + 2) + sum(5 + 2 * (s - 1 - i) for i in range(1, s - 1)) + 5
+
+
+def c2_matrix_leg(window=(300, 1300)):
+    """BRUSS2D N=2000, t in [0, 1], h0 1e-3: {DOPRI54, VERNER65} x tol {1e-6,
+    1e-8} x {full storage D form, low-storage fused PipeDls}; each timed over
+    attempted steps window[0]..window[1] of the solve (differenced, so set-up
+    is excluded), with the algorithmic HBM bytes of the form and the device
+    bytes the solve holds."""
+    import torch
+    import rk_amd
+    p = rk_amd.Bruss2D(N_GRID, 2e-3)
+    y0 = torch.from_numpy(p.ode_start()).cuda()
+    y = torch.empty_like(y0)
+    torch.cuda.synchronize()
+    rows = []
+    for meth in ("DOPRI54", "VERNER65"):
+        s = rk_amd.method(meth).s
+        for tol in (1e-6, 1e-8):
+            for impl in ("D", "PipeDls"):
+                kw = dict(h0=1e-3, impl=impl)
+                a, _ = rk_amd.solve(p, meth, 0.0, 1.0, y0, y, tol, step_limit=window[0], **kw)
+                b, _ = rk_amd.solve(p, meth, 0.0, 1.0, y0, y, tol, step_limit=window[1], **kw)
+                us = (b.kernel_time - a.kernel_time) / (b.steps - a.steps) * 1e6
+                passes = d_form_passes(s) if impl == "D" else 2
+                rows.append({"method": meth, "tol": tol, "impl": impl, "us_per_step": us,
+                             "hbm_gbs_algorithmic": passes * 8.0 * p.n / (us * 1e-6) / 1e9,
+                             "vector_passes_per_step": passes,
+                             "resident_bytes": b.resident_bytes})
+    return rows
+
+
 def batched_leg(sample_every=2):
     """Mixed batch of 512 BRUSS2D instances (N uniform on 32..512 from seed 0,
     alpha log-uniform on [1e-3, 5e-3], noisy u0; DOPRI54 tol 1e-6, t in
@@ -296,6 +333,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)
     if not a.no_batch and a.step_limit == 0:
+        res["c2_matrix"] = c2_matrix_leg()
+        print("c2 matrix done", file=sys.stderr, flush=True)
         res["batched"] = batched_leg()
         print("batched: %.2f s vs plain %.2f s" % (res["batched"]["batched_s"],
                                                    res["batched"]["plain_s"]),

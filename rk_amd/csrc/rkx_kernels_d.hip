@@ -11,11 +11,17 @@
 // dy, err.
 #include "rkx_device.hpp"
 
+#include <algorithm>
+
 namespace rkx {
 
 namespace {
 
 constexpr int kBlock = 256;
+// workgroups of the last stage (grid-stride): 8 per CU of MI355X, enough to
+// stream at full bandwidth, few enough that the per-workgroup atomics on the
+// step state do not serialize
+constexpr int kLastBlocks = 2048;
 
 __device__ __forceinline__ double *ycur(state *st, double *y0, double *y1)
 {
@@ -96,13 +102,15 @@ __global__ __launch_bounds__(kBlock) void k_d_last(dev_ode p, const dev_coef *__
   __shared__ unsigned int zred[kBlock / 64];
   if (st->done | st->pow_wait)
     return;                       // uniform: every block of the grid returns
-  const int cell = blockIdx.x * kBlock + threadIdx.x;
   const int cur = st->cur;
   const double *y = cur ? y1 : y0;
   double *ynew = cur ? y0 : y1;
   double qmax = 0.0;
   unsigned int zero = 0u;
-  if (cell < p.cells) {
+  // grid-stride: a bounded number of workgroups folds its maximum into the
+  // step state (one atomic each), instead of one per 256 cells
+  for (int cell = blockIdx.x * kBlock + threadIdx.x; cell < p.cells;
+       cell += gridDim.x * kBlock) {
     const size_t n = static_cast<size_t>(p.n);
     const int stage = s - 1;
     double F0, F1;
@@ -162,8 +170,8 @@ hipError_t launch_d_step(const dev_ode &p, const dev_coef *m, int s, state *st, 
   for (int i = 1; i < s - 1; ++i)
     hipLaunchKernelGGL(k_d_interm, dim3(grid), dim3(kBlock), 0, stream, p, m, s, i, st, w, dy,
                        err);
-  hipLaunchKernelGGL(k_d_last, dim3(grid), dim3(kBlock), 0, stream, p, m, s, st, y0, y1, w,
-                     dy, err);
+  hipLaunchKernelGGL(k_d_last, dim3(std::min(grid, kLastBlocks)), dim3(kBlock), 0, stream, p, m,
+                     s, st, y0, y1, w, dy, err);
   return hipGetLastError();
 }
 

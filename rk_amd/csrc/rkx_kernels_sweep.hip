@@ -119,7 +119,16 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   double yo[NS];                                 // y_old of finishing rows, ring by iteration
   constexpr int DY = 2;                          // y_old loaded DY iterations ahead
   gconst *yc = y + 2 * ccol + var;
-  auto yrow = [&](int k) -> double { return yc[static_cast<size_t>(reflect(k, N)) * N2]; };
+  // Ghost rows: a single reflection about row 0 / N-1, min(|k|, 2(N-1) - |k|),
+  // covers every row a band loads once the grid is taller than the halo plus
+  // the prefetch; smaller grids take the general loop.  Offsets fit 32 bits
+  // (n = 2N^2 < 2^31).
+  const bool single_reflect = N > s + PF + 3;
+  auto yrow = [&](int k) -> double {
+    const int a = k < 0 ? -k : k;
+    const int kr = single_reflect ? min(a, 2 * (N - 1) - a) : reflect(k, N);
+    return yc[static_cast<unsigned>(kr) * static_cast<unsigned>(N2)];
+  };
   static_for<PF + 2>([&](auto dt) {
     constexpr int d = decltype(dt)::value - 1;   // rows it0-1 .. it0+PF
     yr[(d + NS) % NS] = yrow(it0 + d);
@@ -168,7 +177,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
         // ahead, and y_old of the row the last stage finishes DY iterations
         // ahead (clamped into the band: rows outside it are never stored).
         const int kf = min(max(rr + DY - (s - 1), r0), r1 - 1);
-        yo[(U + DY) % NS] = yc[static_cast<size_t>(kf) * N2];
+        yo[(U + DY) % NS] = yc[static_cast<unsigned>(kf) * static_cast<unsigned>(N2)];
         yr[(U + 1 + PF) % NS] = yrow(rr + 1 + PF);
         const int kl = rr - (s - 1);                 // row finishing this iteration
         const double yold = yo[U % NS];
@@ -242,7 +251,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
             const double ynw = yold + acc[sl][0];
             const bool out = out_col && (kl >= r0);
             if (out)
-              yn[static_cast<size_t>(kl) * N2 + 2 * col + VAR] = ynw;
+              yn[static_cast<unsigned>(kl) * static_cast<unsigned>(N2) + 2 * col + VAR] = ynw;
             // without branches: q = |e| / max(|y_new|, |y_old|); a zero
             // divisor raises the sentinel, NaN quotients are skipped by fmax
             const double dv = fmax(fabs(ynw), fabs(yold));
