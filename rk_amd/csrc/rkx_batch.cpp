@@ -114,7 +114,7 @@ int rebuild_tables(rkx_batch *b)
     g.pattern = I.pattern;
     g.K = I.K;
     const int nt =
-        I.K == 0 ? make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).ntiles
+        I.K == 0 ? make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg
                  : make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).ntiles;
     for (int t = 0; t < nt; ++t)
       g.tiles.push_back(make_int2(i, t));

@@ -76,8 +76,8 @@ int check_ode(const rkx_ode *p)
   case RKX_BRUSS2D:
     if (p->grid < 2 || p->n != 2 * p->grid * p->grid)
       return fail(RKX_EINVAL, "BRUSS2D needs N >= 2 and n = 2*N*N");
-    if (p->grid > 32768)
-      return fail(RKX_ELIMIT, "BRUSS2D grid above 32768 (n must fit in int)");
+    if (p->grid > 32767)
+      return fail(RKX_ELIMIT, "BRUSS2D grid above 32767 (n = 2N^2 must fit in int)");
     break;
   case RKX_LV:
     if (p->grid < 1 || p->n != 2 * p->grid)
@@ -264,8 +264,8 @@ void rkx_opts_default(rkx_opts *o)
 
 int rkx_ode_bruss2d(rkx_ode *p, int N, double alpha)
 {
-  if (!p || N < 2 || N > 32768)
-    return fail(RKX_EINVAL, "rkx_ode_bruss2d: need 2 <= N <= 32768");
+  if (!p || N < 2 || N > 32767)
+    return fail(RKX_EINVAL, "rkx_ode_bruss2d: need 2 <= N <= 32767 (n = 2N^2 fits in int)");
   std::memset(p, 0, sizeof(*p));
   p->kind = RKX_BRUSS2D;
   p->grid = N;
@@ -544,7 +544,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     if (int rc = mem.alloc(reinterpret_cast<void **>(&chain_dev), sizeof(chain_inst)))
       return rc;
     const chain_inst hi = make_chain_inst(dp, *m, st_dev, cf_dev, ybuf[0], ybuf[1]);
-    ntiles = hi.ntiles;
+    ntiles = hi.nwg;
     RKX_HIP(hipMemcpyAsync(chain_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));
     RKX_HIP(hipStreamSynchronize(stream));
   } else if (impl != RKX_IMPL_D) {

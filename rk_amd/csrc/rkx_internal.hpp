@@ -225,7 +225,8 @@ hipError_t launch_sweep(int pattern, int K, const sweep_inst *insts, const int2 
 struct chain_inst {
   int cells;             // M
   int tile_out;          // output cells per segment
-  int ntiles;            // segments (workgroups per step)
+  int ntiles;            // segments of 1024 cells per step
+  int nwg;               // workgroups per step (each loops over segments)
   double c0, d0, d1, c1, diff;
   double *y0, *y1;
   state *st;

@@ -11,6 +11,9 @@
 // valid region shrinks by one cell), so a segment writes T - 2s cells of
 // y_new.  Periodic ghost cells are loaded by index wrap.  Per step the kernel
 // reads y once (plus the halo) and writes y_new once: two vectors resident.
+// An instance gets at most kChainWG workgroups, each looping over segments,
+// so the per-workgroup fold of the error maximum into the instance's state
+// (atomics on one cache line) stays short for long chains.
 #include "rkx_patterns.hpp"
 
 namespace rkx {
@@ -19,6 +22,7 @@ namespace {
 constexpr int kCB = 256;     // lanes per workgroup
 constexpr int kCPT = 4;      // cells per lane
 constexpr int kT = kCB * kCPT;
+constexpr int kChainWG = 512;  // workgroups per instance and step
 
 __device__ __forceinline__ int wrap(long long c, int M)
 {
@@ -58,8 +62,6 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
   const dev_coef *__restrict__ cf = I.cf;
 
   const int t = threadIdx.x;
-  const long long out0 = static_cast<long long>(td.y) * I.tile_out;   // first output cell
-  const long long c0 = out0 - s + static_cast<long long>(t) * C;      // this lane's first cell
 
   double kv[s][s + 1];
   static_for<s>([&](auto it) {
@@ -76,6 +78,11 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
     kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(cf->col[i][s - i]) : 0.0;
   });
 
+  double qmax = 0.0;
+  unsigned int zero = 0u;
+  for (int seg = td.y; seg < I.ntiles; seg += I.nwg) {
+  const long long out0 = static_cast<long long>(seg) * I.tile_out;    // first output cell
+  const long long c0 = out0 - s + static_cast<long long>(t) * C;      // this lane's first cell
   double2 yv[C];
   double2 acc[C][s + 1];                 // [0] dy, [1..s-1] w_l, [s] err
   static_for<C>([&](auto ct) {
@@ -83,8 +90,6 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
     yv[c] = y[wrap(c0 + c, M)];
   });
 
-  double qmax = 0.0;
-  unsigned int zero = 0u;
   static_for<s>([&](auto itg) {
     constexpr int i = decltype(itg)::value;
     double2 cv[C];
@@ -151,7 +156,9 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
       }
     });
   });
-  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.ntiles));
+  __syncthreads();                       // the next segment reuses the edge buffers
+  }
+  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.nwg));
 }
 
 }  // namespace
@@ -168,6 +175,7 @@ chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, con
   I.cells = p.cells;
   I.tile_out = kT - 2 * m.s;
   I.ntiles = (p.cells + I.tile_out - 1) / I.tile_out;
+  I.nwg = I.ntiles < kChainWG ? I.ntiles : kChainWG;
   I.c0 = p.c0;
   I.d0 = p.d0;
   I.d1 = p.d1;

@@ -103,3 +103,13 @@ def test_serving_summary_statistics():
     assert s["requests"] == 100 and s["solves_per_s"] == 0.5
     assert abs(s["p50_latency_s"] - 0.505) < 1e-12
     assert abs(s["p99_latency_s"] - np.percentile(np.arange(1, 101) * 0.01, 99)) < 1e-12
+
+
+def test_bruss2d_size_limit():
+    """n = 2N^2 must fit a C int: N = 32767 is the largest grid (descriptor only)."""
+    import rk_amd
+    assert rk_amd.Bruss2D(32767).n == 2 * 32767 * 32767
+    with pytest.raises(rk_amd.RkxError):
+        rk_amd.Bruss2D(32768)
+    with pytest.raises(rk_amd.RkxError):
+        rk_amd.Bruss2D(1)
