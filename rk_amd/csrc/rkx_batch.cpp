@@ -35,7 +35,8 @@ struct batch_inst {
   rkx_method m{};
   dev_ode dp{};
   dev_method dm{};
-  sweep_cfg cfg{1, 1};
+  sweep_cfg base{1, 1};                 // tile shape chosen at admission
+  sweep_cfg cfg{1, 1};                  // current shape (bands shortened as the batch drains)
   double *y[2] = {nullptr, nullptr};
   double *w = nullptr, *dy = nullptr, *err = nullptr;
   size_t bytes = 0;
@@ -67,6 +68,7 @@ struct rkx_batch {
   hipGraph_t graph = nullptr;           // kPollSteps batched steps, captured once per table set
   hipGraphExec_t graph_exec = nullptr;
   bool graph_ok = false;
+  int cus = 256;                        // compute units of the device
   std::vector<batch_inst> inst;
   std::map<std::pair<int, int>, tile_group> groups;
   size_t bytes = 0;
@@ -102,8 +104,51 @@ sweep_cfg batch_cfg(const dev_ode &p, const rkx_method &m, const rkx_batch_opts 
   return c;
 }
 
+// Band height as the batch drains: the admitted shape keeps halo rows a small
+// share while many instances fill the CUs; once fewer tiles than two per CU
+// remain, the remaining fused instances get bands of half, a quarter, ...
+// (down to 16 rows) so that a lone large instance still spreads over the
+// chip instead of one workgroup sweeping all its rows.  The tiling never
+// changes an instance's arithmetic, only how its rows are shared out.
+int adapt_bands(rkx_batch *b)
+{
+  auto tiles_at = [&](int div) {
+    long long t = 0;
+    for (int i = 0; i < b->hi_slot; ++i) {
+      const batch_inst &I = b->inst[static_cast<size_t>(i)];
+      if (!I.used || I.done || I.storage != RKX_STORAGE_LOW)
+        continue;
+      if (I.K == 0) {
+        t += make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg;
+      } else {
+        const sweep_cfg c{I.base.waves, std::max(16, I.base.rows / div)};
+        t += make_sweep_inst(I.dp, I.m, c, nullptr, nullptr, nullptr, nullptr).ntiles;
+      }
+    }
+    return t;
+  };
+  int div = 1;
+  while (div < 64 && tiles_at(div) < 2LL * b->cus && tiles_at(2 * div) > tiles_at(div))
+    div *= 2;
+  for (int i = 0; i < b->hi_slot; ++i) {
+    batch_inst &I = b->inst[static_cast<size_t>(i)];
+    if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K == 0)
+      continue;
+    const sweep_cfg c{I.base.waves, std::max(16, I.base.rows / div)};
+    if (c.rows == I.cfg.rows)
+      continue;
+    I.cfg = c;
+    const sweep_inst si = make_sweep_inst(I.dp, I.m, c, b->st_dev + i, b->cf_dev + i, I.y[0],
+                                          I.y[1]);
+    RKX_HIP(hipMemcpyAsync(b->si_dev + i, &si, sizeof(si), hipMemcpyHostToDevice, b->stream));
+  }
+  return RKX_OK;
+}
+
 int rebuild_tables(rkx_batch *b)
 {
+  if (int rc = adapt_bands(b))
+    return rc;
   for (auto &kv : b->groups)
     kv.second.tiles.clear();
   for (int i = 0; i < b->hi_slot; ++i) {
@@ -336,6 +381,13 @@ int rkx_batch_create(rkx_batch **out, const rkx_batch_opts *o)
     return rc;
   }
   b->inst.resize(kBatchCap);
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0)
+    b->cus = cus;
+  else
+    (void) hipGetLastError();
   *out = b;
   return RKX_OK;
 }
@@ -472,6 +524,7 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
   } else if (storage == RKX_STORAGE_LOW) {
     I.pattern = sweep_pattern(*m);
     I.cfg = batch_cfg(dp, *m, b->opts);
+    I.base = I.cfg;
     I.K = I.cfg.waves;
     const sweep_inst si =
         make_sweep_inst(dp, *m, I.cfg, b->st_dev + slot, b->cf_dev + slot, I.y[0], I.y[1]);
