@@ -113,3 +113,46 @@ def test_bruss2d_size_limit():
         rk_amd.Bruss2D(32768)
     with pytest.raises(rk_amd.RkxError):
         rk_amd.Bruss2D(1)
+
+
+def test_solve_argument_errors_before_any_device_call():
+    """rkx_solve / rkx_batch_* validate their arguments on the host and return
+    RKX_EINVAL (with a message) before touching the device: a non-explicit
+    table, unknown impl / step-control modes, null vectors, a trace length
+    without a buffer, a batch with an unsupported control mode."""
+    import ctypes as C
+    import rk_amd
+    from rk_amd._lib import Method, Opts, Stats, lib
+    L = lib()
+    p = rk_amd.Bruss2D(16)
+    m = rk_amd.method("DOPRI54")
+    dummy = C.c_void_p(0x1000)          # never dereferenced: validation fails first
+    st = Stats()
+
+    def solve_rc(meth, opts, y0=dummy, y=dummy):
+        return L.rkx_solve(C.byref(p.desc), C.byref(meth), 0.0, 1.0, y0, y, 1e-6,
+                           C.byref(opts), C.byref(st))
+
+    o = Opts()
+    L.rkx_opts_default(C.byref(o))
+    implicit = Method()
+    C.memmove(C.byref(implicit), C.byref(m), C.sizeof(Method))
+    implicit.A[2][2] = 0.5                       # diagonal entry: not explicit
+    assert solve_rc(implicit, o) == -1 and b"explicit" in L.rkx_last_error()
+    for field, bad in (("impl", 7), ("ctrl", 9)):
+        ob = Opts()
+        L.rkx_opts_default(C.byref(ob))
+        setattr(ob, field, bad)
+        assert solve_rc(m, ob) == -1
+    assert solve_rc(m, o, y0=C.c_void_p(0)) == -1 and b"null" in L.rkx_last_error()
+    ot = Opts()
+    L.rkx_opts_default(C.byref(ot))
+    ot.trace_cap = 10                             # no trace buffer
+    assert solve_rc(m, ot) == -1 and b"trace" in L.rkx_last_error()
+    from rk_amd.batch import BatchOpts
+    bo = BatchOpts()
+    L.rkx_batch_opts_default(C.byref(bo))
+    bo.ctrl = rk_amd.RKX_CTRL_HOST                # per-step host control: not for batches
+    h = C.c_void_p()
+    assert L.rkx_batch_create(C.byref(h), C.byref(bo)) == -1
+    assert L.rkx_batch_create(None, C.byref(bo)) == -1
