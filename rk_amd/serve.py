@@ -69,6 +69,7 @@ def serve_batched(requests, hbm_budget=64 << 30, poll_steps=64, realtime=True, c
     active = {}                # instance id -> request
     done = []
     nxt = 0
+    peak = 0
     with Batch(hbm_budget=hbm_budget, ctrl=ctrl) as b:
         t_start = time.perf_counter()
         now = lambda: time.perf_counter() - t_start
@@ -86,6 +87,7 @@ def serve_batched(requests, hbm_budget=64 << 30, poll_steps=64, realtime=True, c
                 r.admitted = now()
                 active[i] = r
                 nxt += 1
+                peak = max(peak, len(active))
             if not active:
                 # idle: sleep until the next arrival
                 if nxt < len(pending):
@@ -105,7 +107,9 @@ def serve_batched(requests, hbm_budget=64 << 30, poll_steps=64, realtime=True, c
                     if log:
                         log(r)
         makespan = now()
-    return done, summarize(done, makespan)
+    summ = summarize(done, makespan)
+    summ["peak_concurrent"] = peak
+    return done, summ
 
 
 def serve_plain(requests, realtime_model=True, keep_results=False):
@@ -117,7 +121,6 @@ def serve_plain(requests, realtime_model=True, keep_results=False):
     from .solver import solve
     clock = 0.0
     done = []
-    busy = 0.0
     for r in sorted(requests, key=lambda q: q.arrival):
         y0 = torch.from_numpy(np.ascontiguousarray(r.y0)).cuda()
         y = torch.empty_like(y0)
@@ -126,7 +129,6 @@ def serve_plain(requests, realtime_model=True, keep_results=False):
         st, _ = solve(r.problem, r.method, r.t0, r.te, y0, y, r.tol, h0=r.h0)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
-        busy += dt
         start = max(clock, r.arrival) if realtime_model else clock
         r.admitted = start
         r.finished = start + dt

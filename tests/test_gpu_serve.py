@@ -45,6 +45,8 @@ def test_served_results_equal_plain(rk, realtime, budget):
     done, summ = serve_batched(stream(rk), hbm_budget=budget, poll_steps=16,
                                realtime=realtime, keep_results=True)
     assert summ["requests"] == len(want)
+    if not realtime:
+        assert summ["peak_concurrent"] >= 2
     for r in done:
         y, steps = want[r.rid]
         assert r.steps == steps, r.rid
@@ -65,3 +67,29 @@ def test_budget_limits_resident_set(rk):
     assert len(done) == len(reqs)
     # some request had to wait for a retirement
     assert max(r.admitted for r in done) > min(r.finished for r in done)
+
+
+def test_requests_join_a_running_batch(rk):
+    """A long request starts alone; short ones arriving while it runs are
+    admitted into the running batch (continuous batching, not a queue) and
+    finish before it, with results equal to their plain solves."""
+    from rk_amd.serve import Request, serve_batched, serve_plain
+    def reqs():
+        out = []
+        p = rk.Bruss2D(400, 5e-3)          # about 8000 steps: runs for tenths of a second
+        out.append(Request(0, 0.0, p, "DOPRI54", 0.0, 4.0, p.ode_start_seeded(5, 0, 0.01), 1e-6,
+                           h0=1e-3))
+        for k in range(1, 6):
+            q = rk.Bruss2D(24 + 8 * k, 2e-3)
+            out.append(Request(k, 0.02 * k, q, "DOPRI54", 0.0, 0.3, q.ode_start_seeded(5, k, 0.01),
+                               1e-6, h0=1e-3))
+        return out
+    want, _ = serve_plain(reqs(), keep_results=True)
+    want = {r.rid: r.result for r in want}
+    done, summ = serve_batched(reqs(), poll_steps=16, realtime=True, keep_results=True)
+    by = {r.rid: r for r in done}
+    assert summ["peak_concurrent"] >= 2
+    assert all(by[k].finished < by[0].finished for k in range(1, 6))
+    assert all(by[k].admitted < by[0].finished for k in range(1, 6))
+    for k, r in by.items():
+        assert np.array_equal(r.result, want[k]), k
