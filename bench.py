@@ -255,8 +255,9 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (a single "
+              "grid would shard by row bands with one halo exchange and one max all-reduce "
+              "per step; batched instances would shard whole); use --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
