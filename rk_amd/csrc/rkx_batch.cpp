@@ -504,6 +504,7 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
   hs.ord = m->ord;
   hs.ctrl = b->opts.ctrl;
   hs.s = m->s;
+  hs.pow_spin = pow_spin_limit();
   hs.meth = b->dm_dev + slot;
   hs.coef = b->cf_dev + slot;
   hs.done = !(t0 < te);

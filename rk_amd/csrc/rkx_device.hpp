@@ -135,10 +135,6 @@ __device__ __forceinline__ void err_term(double e, double yn, double yo, double 
 }
 
 // ---- step control (include/tools.h:217-278, 191-198) ------------------------
-// Iterations of the bounded wait for the host pow service (each one sleeps and
-// reads the mailbox over the host link, roughly a microsecond): about 0.2 s
-// in total, after which the step is handed to the host loop instead.
-constexpr int kPowSpin = 200000;
 
 // Posts pow(st->pow_x, st->pow_y) to the host service and waits, bounded, for
 // the answer.  Returns false on time-out.
@@ -154,7 +150,8 @@ __device__ inline bool pow_service(state *st, double &p)
                      static_cast<unsigned long long>(__double_as_longlong(st->pow_y)),
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&mb->req, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  for (int it = 0; it < kPowSpin; ++it) {
+  const int spin = st->pow_spin;
+  for (int it = 0; it < spin; ++it) {
     if (__hip_atomic_load(&mb->resp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == seq) {
       const unsigned long long pb = __hip_atomic_load(
           reinterpret_cast<unsigned long long *>(&mb->p), __ATOMIC_RELAXED,

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <atomic>
 #include <mutex>
@@ -114,6 +115,22 @@ void fill_dev_method(const rkx_method &m, dev_method &d)
     d.e[i] = m.b[i] - m.b_hat[i];      // seq/D.c:40-41
     d.c[i] = m.c[i];
   }
+}
+
+// Iterations of the bounded wait for the host pow service (each one sleeps and
+// reads the mailbox over the host link, roughly a microsecond): about 0.2 s
+// in total, after which the step is handed to the host loop instead.
+constexpr int kPowSpin = 200000;
+
+int pow_spin_limit()
+{
+  const char *e = std::getenv("RKX_POW_SPIN");
+  if (e && *e) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 0 && v <= kPowSpin)
+      return static_cast<int>(v);
+  }
+  return kPowSpin;
 }
 
 // initial_stepsize (src/tools.c:25-36)
@@ -507,6 +524,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   hs.ctrl = o.ctrl;
   hs.trace = trace_dev;
   hs.trace_cap = o.trace_cap;
+  hs.pow_spin = pow_spin_limit();
   hs.s = s;
   hs.meth = dm_dev;
   hs.coef = cf_dev;

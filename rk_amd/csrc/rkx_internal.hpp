@@ -87,6 +87,7 @@ struct alignas(64) state {
   int pow_wait;                      // 1: the new h waits for pow(pow_x, pow_y) from the host
   int trace_cap;
   int s;                             // stages
+  int pow_spin;                      // bound of the wait for the host pow service (iterations)
   double pow_x, pow_y;               // pending pow arguments
   unsigned long long pow_seq;        // last request number posted to the mailbox
   pow_mailbox *mb;                   // host pow service mailbox (device-visible pointer)
@@ -153,6 +154,10 @@ int check_ode(const rkx_ode *p);
 int check_method(const rkx_method *m);
 void fill_dev_method(const rkx_method &m, dev_method &d);
 double initial_h(double H, double tol, int limit, double h0);   // src/tools.c:25-36
+// Iterations the device step control waits for the host pow service before
+// handing the step to the host (RKX_POW_SPIN overrides; 0 makes every step
+// take the host path, which tests that path).
+int pow_spin_limit();
 int default_stream(hipStream_t *out);
 // the host half of the step control (ctrl_decide/ctrl_finish with the C
 // library's pow) and the next step's premultiplied coefficients

@@ -252,3 +252,25 @@ def test_step_limit(rk):
     p = rk.Bruss2D(40)
     _, st, _ = run_gpu(rk, p, "DOPRI54", 0.0, 4.0, p.ode_start(), 1e-6, h0=1e-3, step_limit=37)
     assert st.steps == 37
+
+
+def test_pow_service_timeout_path(rk, orc, monkeypatch):
+    """Every step's pow request times out (RKX_POW_SPIN=0): the kernels set
+    pow_wait, the rest of the chunk returns at once, and the host finishes
+    the step with the C library's pow -- results stay bitwise equal to the
+    oracle, through the single-solve loop and the batch engine."""
+    monkeypatch.setenv("RKX_POW_SPIN", "0")
+    N = 40
+    po = orc.make_ode("bruss2d", N)
+    y0 = orc.ode_start(po)
+    yo, so, _ = orc.solve(po, "DOPRI54", 0.0, 0.5, y0, 1e-6, h0=1e-3)
+    p = rk.Bruss2D(N)
+    for impl in ("PipeDls", "D"):
+        y, st, _ = run_gpu(rk, p, "DOPRI54", 0.0, 0.5, y0, 1e-6, h0=1e-3, impl=impl)
+        assert (st.steps_acc, st.steps_rej) == (so["acc"], so["rej"])
+        assert np.array_equal(y, yo), impl
+    with rk.Batch() as b:
+        ids = [b.add(p, "DOPRI54", 0.0, 0.5, y0, 1e-6, h0=1e-3) for _ in range(3)]
+        assert b.run(100000) == 0
+        for i in ids:
+            assert np.array_equal(b.result(i), yo)
