@@ -241,6 +241,19 @@ def test_c1_accuracy_against_long_double(rk, orc, meth, tol):
     assert err <= 10 * tol
 
 
+def test_c2_full_interval_d_form(rk):
+    """Config 2 (DOPRI54, tol 1e-6) through the full-storage stage kernels of
+    the D form over the whole interval: the reference's step counts and
+    footprint bitwise (19 743 steps, about 19 s)."""
+    case = [c for c in KAT if c["name"] == "c2_bruss2000_dopri54_tol1e-6"][0]
+    p = rk.Bruss2D(case["grid"], case["alpha"])
+    y, st, _ = run_gpu(rk, p, case["method"], case["t0"], case["te"], p.ode_start(), case["tol"],
+                       h0=case["h0"], impl="D")
+    assert (st.steps_acc, st.steps_rej) == (case["acc"], case["rej"])
+    assert [y[0], y[-1], y.min(), y.max()] == [float(x) for x in case["footprint"]]
+    assert st.resident_bytes > 9 * p.n * 8           # full storage: y x2, w_1..w_6, dy, err
+
+
 def test_zero_length_interval(rk):
     p = rk.Bruss2D(8)
     y0 = p.ode_start()
