@@ -115,7 +115,8 @@ typedef struct rkx_opts {
   int step_limit;        /* STEP_LIMIT (config.h:70-72); <= 0: unlimited */
   int impl;              /* RKX_IMPL_* */
   int ctrl;              /* RKX_CTRL_* */
-  int tile_rows;         /* sweep band height in grid rows (0: tuned default) */
+  int tile_rows;         /* sweep band height in grid rows (0: tuned default; VERNER65
+                            rounds it up so that a band fills whole trips of its ring) */
   int tile_waves;        /* sweep strip width in wavefronts (0: tuned default) */
   double *trace;         /* optional host buffer, 3 doubles (t, h, err) per attempt */
   int trace_cap;         /* attempts the trace buffer holds */
@@ -163,7 +164,8 @@ typedef struct rkx_batch_opts {
   int ctrl;               /* RKX_CTRL_DEVICE or RKX_CTRL_DEVICE_POW */
   int tile_waves;         /* fused-kernel strip width for all instances (0: chosen per size;
                              rk_amd/autotune.py picks both by timing the batch's own work) */
-  int tile_rows;          /* fused-kernel band height (0: chosen per size) */
+  int tile_rows;          /* fused-kernel band height (0: chosen per size; rounded up as
+                             rkx_opts.tile_rows) */
   void *stream;           /* hipStream_t, NULL: the library's own stream */
 } rkx_batch_opts;
 

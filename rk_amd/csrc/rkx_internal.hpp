@@ -218,8 +218,13 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m);
 // Tile-shape cost model: relative cost of one lane x row iteration for strips
 // of K = 1..4 wavefronts per component (f[K-1]); f == nullptr or f[K-1] <= 0
 // selects the measured defaults.  Returns the modelled cost of one step of an
-// N x N instance with strips of 64K columns and bands of R rows.
-double sweep_tile_cost(int N, int s, int K, int R, const double *f);
+// N x N instance with strips of 64K columns and bands of R rows; trip > 0
+// rounds a band's row iterations up to whole trips of that many rows.
+double sweep_tile_cost(int N, int s, int K, int R, const double *f, int trip);
+// Band height a tile gets for `rows` requested rows: unchanged, or for tables
+// whose fused kernel runs whole trips of its s+1-row ring, rounded up so that
+// the band's rows + 2(s-1) row iterations fill whole trips.
+int sweep_band_rows(int pattern, int s, int rows);
 sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
                            const dev_coef *cf, double *y0, double *y1);
 // tiles == nullptr: a single instance (insts[0]) with ntiles tiles

@@ -121,9 +121,10 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   gconst *yc = y + 2 * ccol + var;
   // Ghost rows: a single reflection about row 0 / N-1, min(|k|, 2(N-1) - |k|),
   // covers every row a band loads once the grid is taller than the halo plus
-  // the prefetch; smaller grids take the general loop.  Offsets fit 32 bits
-  // (n = 2N^2 < 2^31).
-  const bool single_reflect = N > s + PF + 3;
+  // the prefetch (plus one trip of the ring with full trips, which may run
+  // past the band's last iteration); smaller grids take the general loop.
+  // Offsets fit 32 bits (n = 2N^2 < 2^31).
+  const bool single_reflect = N > s + PF + 3 + (P::full_trips ? NS : 0);
   auto yrow = [&](int k) -> double {
     const int a = k < 0 ? -k : k;
     const int kr = single_reflect ? min(a, 2 * (N - 1) - a) : reflect(k, N);
@@ -170,8 +171,10 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
       static_for<NS>([&](auto ut) {
         constexpr int U = decltype(ut)::value;
         const int rr = rb + U;
-        if (rr >= it1)
-          return;
+        if constexpr (!P::full_trips) {
+          if (rr >= it1)
+            return;
+        }
         // Loads complete in issue order, so every load used in this
         // iteration was issued in an earlier one: y for stage 0 PF rows
         // ahead, and y_old of the row the last stage finishes DY iterations
@@ -184,7 +187,9 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
 
         // Every stage runs every iteration: during the first 2(s-1)
         // iterations of a band, stage i works on rows above its range, whose
-        // results are never read by an in-range row and never stored.
+        // results are never read by an in-range row and never stored; with
+        // full trips, iterations past it1 work on rows below the band, which
+        // are never stored either.
         const int buf = (rr - it0) & 1;
         static_for<s>([&](auto itg) {
           constexpr int i = decltype(itg)::value;
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
           if constexpr (i == s - 1) {
             // y_new = y + dy; the error term against y_old (block-inline.h:298-320)
             const double ynw = yold + acc[sl][0];
-            const bool out = out_col && (kl >= r0);
+            const bool out = out_col && (kl >= r0) && (!P::full_trips || kl < r1);
             if (out)
               yn[static_cast<unsigned>(kl) * static_cast<unsigned>(N2) + 2 * col + VAR] = ynw;
             // without branches: q = |e| / max(|y_new|, |y_old|); a zero
@@ -338,7 +343,7 @@ bool sweep_supported(const dev_ode &p, const rkx_method &m)
 // (6 per workgroup) do not pack two workgroups per CU.
 constexpr double kTileCost[4] = {1.36, 1.0, 1.67, 0.97};
 
-double sweep_tile_cost(int N, int s, int K, int R, const double *f)
+double sweep_tile_cost(int N, int s, int K, int R, const double *f, int trip)
 {
   const int L = 64 * K;
   const int oc = L - 2 * s;
@@ -348,9 +353,37 @@ double sweep_tile_cost(int N, int s, int K, int R, const double *f)
   const long long nstrips = (N + oc - 1) / oc;
   const long long nbands = (N + R - 1) / R;
   const long long reff = (N + nbands - 1) / nbands;
+  long long iters = reff + 2 * (s - 1);
+  if (trip > 0)
+    iters = (iters + trip - 1) / trip * trip;
   // + 4: the prologue of a tile (coefficients, first rows) in row iterations
   return fk * static_cast<double>(nstrips * L) * static_cast<double>(nbands) *
-         static_cast<double>(reff + 2 * (s - 1) + 4);
+         static_cast<double>(iters + 4);
+}
+
+namespace {
+
+bool full_trips_of(int pattern)
+{
+  switch (pattern) {
+  case 1: return pat_dopri54::full_trips;
+  case 2: return pat_verner65::full_trips;
+  case 3: return pat_rkf23::full_trips;
+  case 4: return pat_heun2::full_trips;
+  case 5: return pat_rk4::full_trips;
+  default: return pat_ssprk3::full_trips;
+  }
+}
+
+}  // namespace
+
+int sweep_band_rows(int pattern, int s, int rows)
+{
+  if (!full_trips_of(pattern))
+    return rows;
+  const int ns = s + 1;
+  const int it = rows + 2 * (s - 1);
+  return (it + ns - 1) / ns * ns - 2 * (s - 1);
 }
 
 // Default tile: strips of 64*K columns (2K waves per workgroup) and the band
@@ -385,7 +418,9 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     int rows = (N + bands - 1) / bands;
     if (rows < 4)
       rows = 4;
-    const double cost = sweep_tile_cost(N, s, K, rows, nullptr);
+    rows = sweep_band_rows(pattern_of(m), s, rows);
+    const double cost = sweep_tile_cost(N, s, K, rows, nullptr,
+                                        full_trips_of(pattern_of(m)) ? s + 1 : 0);
     if (cost < best_cost) {
       best_cost = cost;
       best = sweep_cfg{K, rows};
@@ -402,10 +437,10 @@ sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cf
   const int L = 64 * c.waves;
   const int oc_max = L - 2 * m.s;
   I.N = p.N;
-  I.R = c.rows;
+  I.R = sweep_band_rows(pattern_of(m), m.s, c.rows);
   I.nstrips = (p.N + oc_max - 1) / oc_max;
   I.OC = (p.N + I.nstrips - 1) / I.nstrips;
-  I.nbands = (p.N + c.rows - 1) / c.rows;
+  I.nbands = (p.N + I.R - 1) / I.R;
   I.ntiles = I.nstrips * I.nbands;
   I.coef = p.coef;
   I.y0 = y0;

@@ -9,9 +9,16 @@
 
 namespace rkx {
 
-// Zero patterns of the compiled methods (bit i of row l: A[l][i] != 0), and
-// the occupancy (waves per SIMD) the fused kernels are register-allocated for.
+// Zero patterns of the compiled methods (bit i of row l: A[l][i] != 0), the
+// occupancy (waves per SIMD) the fused kernels are register-allocated for,
+// and whether the fused kernel runs whole trips of its unrolled row ring
+// (full_trips: no exit inside a trip, band heights rounded so that a band's
+// row iterations fill whole trips).  Exits inside a trip keep more values
+// live: VERNER65 needs 209 VGPRs without them and more than 256 with them,
+// i.e. two waves per SIMD instead of one; DOPRI54 fits two waves either way
+// and keeps the exits, which spare it the rounding of its band heights.
 struct pat_dopri54 {
+  static constexpr bool full_trips = false;
   static constexpr int occ = 2;
   static constexpr int s = 7;
   static constexpr unsigned rowA[7] = {0x0, 0x1, 0x3, 0x7, 0xF, 0x1F, 0x3D};
@@ -19,13 +26,15 @@ struct pat_dopri54 {
   static constexpr bool fsal = true;      // A[6][j] == b[j]
 };
 struct pat_verner65 {
-  static constexpr int occ = 1;
+  static constexpr bool full_trips = true;
+  static constexpr int occ = 2;
   static constexpr int s = 8;
   static constexpr unsigned rowA[8] = {0x0, 0x1, 0x3, 0x7, 0xF, 0x1F, 0x1F, 0x5F};
   static constexpr unsigned b = 0x3D, e = 0xFD;
   static constexpr bool fsal = false;
 };
 struct pat_rkf23 {
+  static constexpr bool full_trips = false;
   static constexpr int occ = 2;
   static constexpr int s = 3;
   static constexpr unsigned rowA[3] = {0x0, 0x1, 0x3};
@@ -33,6 +42,7 @@ struct pat_rkf23 {
   static constexpr bool fsal = false;
 };
 struct pat_heun2 {
+  static constexpr bool full_trips = false;
   static constexpr int occ = 2;
   static constexpr int s = 2;
   static constexpr unsigned rowA[2] = {0x0, 0x1};
@@ -40,6 +50,7 @@ struct pat_heun2 {
   static constexpr bool fsal = false;
 };
 struct pat_rk4 {
+  static constexpr bool full_trips = false;
   static constexpr int occ = 2;
   static constexpr int s = 4;
   static constexpr unsigned rowA[4] = {0x0, 0x1, 0x2, 0x4};
@@ -47,6 +58,7 @@ struct pat_rk4 {
   static constexpr bool fsal = false;
 };
 struct pat_ssprk3 {
+  static constexpr bool full_trips = false;
   static constexpr int occ = 2;
   static constexpr int s = 3;
   static constexpr unsigned rowA[3] = {0x0, 0x1, 0x3};
