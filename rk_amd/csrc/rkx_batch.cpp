@@ -45,6 +45,7 @@ struct batch_inst {
 
 struct tile_group {
   int pattern = 0, K = 0;
+  bool gen = false;      // some instance needs the general ghost-row form
   std::vector<int2> tiles;
   int2 *dev = nullptr;
   size_t dev_cap = 0;
@@ -149,8 +150,10 @@ int rebuild_tables(rkx_batch *b)
 {
   if (int rc = adapt_bands(b))
     return rc;
-  for (auto &kv : b->groups)
+  for (auto &kv : b->groups) {
     kv.second.tiles.clear();
+    kv.second.gen = false;
+  }
   for (int i = 0; i < b->hi_slot; ++i) {
     const batch_inst &I = b->inst[static_cast<size_t>(i)];
     if (!I.used || I.done || I.storage != RKX_STORAGE_LOW)
@@ -158,6 +161,8 @@ int rebuild_tables(rkx_batch *b)
     tile_group &g = b->groups[{I.pattern, I.K}];
     g.pattern = I.pattern;
     g.K = I.K;
+    if (I.K != 0 && !sweep_single_reflect(I.pattern, I.m.s, I.dp.N))
+      g.gen = true;
     const int nt =
         I.K == 0 ? make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg
                  : make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).ntiles;
@@ -196,8 +201,8 @@ int enqueue_batched_step(rkx_batch *b)
       RKX_HIP(launch_chain(g.pattern, b->ci_dev, g.dev, static_cast<int>(g.tiles.size()),
                            b->stream));
     else
-      RKX_HIP(launch_sweep(g.pattern, g.K, b->si_dev, g.dev, static_cast<int>(g.tiles.size()),
-                           b->stream));
+      RKX_HIP(launch_sweep(g.pattern, g.K, g.gen, b->si_dev, g.dev,
+                           static_cast<int>(g.tiles.size()), b->stream));
   }
   for (int i = 0; i < b->hi_slot; ++i) {
     batch_inst &I = b->inst[static_cast<size_t>(i)];

@@ -579,7 +579,8 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
       return launch_d_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], w, dy, err, stream);
     if (chain)
       return launch_chain(pattern, chain_dev, nullptr, ntiles, stream);
-    return launch_sweep(pattern, cfg.waves, inst_dev, nullptr, ntiles, stream);
+    return launch_sweep(pattern, cfg.waves, !sweep_single_reflect(pattern, s, dp.N), inst_dev,
+                        nullptr, ntiles, stream);
   };
 
   state *hs_pinned = nullptr;   // [0], [1]: snapshots of two chunks in flight

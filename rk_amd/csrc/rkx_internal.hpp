@@ -225,10 +225,15 @@ double sweep_tile_cost(int N, int s, int K, int R, const double *f, int trip);
 // whose fused kernel runs whole trips of its s+1-row ring, rounded up so that
 // the band's rows + 2(s-1) row iterations fill whole trips.
 int sweep_band_rows(int pattern, int s, int rows);
+// Whether every ghost row a band of an N-row grid loads lies within a single
+// reflection about row 0 / N-1 (the kernel's fast form); false for grids
+// shorter than the halo, prefetch and ring: those launch the general form.
+bool sweep_single_reflect(int pattern, int s, int N);
 sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
                            const dev_coef *cf, double *y0, double *y1);
-// tiles == nullptr: a single instance (insts[0]) with ntiles tiles
-hipError_t launch_sweep(int pattern, int K, const sweep_inst *insts, const int2 *tiles,
+// tiles == nullptr: a single instance (insts[0]) with ntiles tiles; gen: the
+// general reflection form (some instance fails sweep_single_reflect)
+hipError_t launch_sweep(int pattern, int K, bool gen, const sweep_inst *insts, const int2 *tiles,
                         int ntiles, hipStream_t stream);
 
 // One LV chain instance as the fused chain kernel sees it (device memory).

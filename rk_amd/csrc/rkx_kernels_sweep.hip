@@ -50,7 +50,9 @@ __device__ __forceinline__ int reflect(int k, int N)
   return k;
 }
 
-template <class P, int K>
+// GEN: ghost rows by the general reflection loop (grids too short for
+// sweep_single_reflect); otherwise by a single reflection.
+template <class P, int K, bool GEN>
 __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ))) void k_sweep(const sweep_inst *__restrict__ insts,
                                                    const int2 *__restrict__ tiles, int ntiles)
 {
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   constexpr int NS = s + 1;          // row slots in the register ring
   constexpr int LP = L + 1;          // + a zero slot
   constexpr int ZERO = L;
-  constexpr int PF = (s - 2 < 2) ? s - 2 : 2;   // y rows loaded ahead of stage 0
+  constexpr int PF = prefetch_rows(s);         // y rows loaded ahead of stage 0
   __shared__ double xch[2][s][2][LP];
   __shared__ double red[2 * K];
   __shared__ unsigned int zred[2 * K];
@@ -122,12 +124,11 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   // Ghost rows: a single reflection about row 0 / N-1, min(|k|, 2(N-1) - |k|),
   // covers every row a band loads once the grid is taller than the halo plus
   // the prefetch (plus one trip of the ring with full trips, which may run
-  // past the band's last iteration); smaller grids take the general loop.
-  // Offsets fit 32 bits (n = 2N^2 < 2^31).
-  const bool single_reflect = N > s + PF + 3 + (P::full_trips ? NS : 0);
+  // past the band's last iteration; sweep_single_reflect); shorter grids take
+  // the general loop (GEN).  Offsets fit 32 bits (n = 2N^2 < 2^31).
   auto yrow = [&](int k) -> double {
     const int a = k < 0 ? -k : k;
-    const int kr = single_reflect ? min(a, 2 * (N - 1) - a) : reflect(k, N);
+    const int kr = GEN ? reflect(k, N) : min(a, 2 * (N - 1) - a);
     return yc[static_cast<unsigned>(kr) * static_cast<unsigned>(N2)];
   };
   static_for<PF + 2>([&](auto dt) {
@@ -277,22 +278,27 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
 }
 
 template <class P, int K>
-hipError_t launch_one(const sweep_inst *insts, const int2 *tiles, int ntiles, hipStream_t stream)
+hipError_t launch_one(bool gen, const sweep_inst *insts, const int2 *tiles, int ntiles,
+                      hipStream_t stream)
 {
-  hipLaunchKernelGGL((k_sweep<P, K>), dim3(ntiles), dim3(128 * K), 0, stream, insts, tiles,
-                     ntiles);
+  if (gen)
+    hipLaunchKernelGGL((k_sweep<P, K, true>), dim3(ntiles), dim3(128 * K), 0, stream, insts,
+                       tiles, ntiles);
+  else
+    hipLaunchKernelGGL((k_sweep<P, K, false>), dim3(ntiles), dim3(128 * K), 0, stream, insts,
+                       tiles, ntiles);
   return hipGetLastError();
 }
 
 template <class P>
-hipError_t launch_pat(int K, const sweep_inst *insts, const int2 *tiles, int ntiles,
+hipError_t launch_pat(int K, bool gen, const sweep_inst *insts, const int2 *tiles, int ntiles,
                       hipStream_t stream)
 {
   switch (K) {
-  case 1: return launch_one<P, 1>(insts, tiles, ntiles, stream);
-  case 2: return launch_one<P, 2>(insts, tiles, ntiles, stream);
-  case 3: return launch_one<P, 3>(insts, tiles, ntiles, stream);
-  default: return launch_one<P, 4>(insts, tiles, ntiles, stream);
+  case 1: return launch_one<P, 1>(gen, insts, tiles, ntiles, stream);
+  case 2: return launch_one<P, 2>(gen, insts, tiles, ntiles, stream);
+  case 3: return launch_one<P, 3>(gen, insts, tiles, ntiles, stream);
+  default: return launch_one<P, 4>(gen, insts, tiles, ntiles, stream);
   }
 }
 
@@ -300,7 +306,8 @@ template <class P, int K>
 int blocks_per_cu()
 {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<P, K>, 128 * K, 0) != hipSuccess) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<P, K, false>, 128 * K, 0) !=
+      hipSuccess) {
     (void) hipGetLastError();
     nb = 1;
   }
@@ -377,6 +384,11 @@ bool full_trips_of(int pattern)
 
 }  // namespace
 
+bool sweep_single_reflect(int pattern, int s, int N)
+{
+  return N > s + prefetch_rows(s) + 3 + (full_trips_of(pattern) ? s + 1 : 0);
+}
+
 int sweep_band_rows(int pattern, int s, int rows)
 {
   if (!full_trips_of(pattern))
@@ -452,18 +464,18 @@ sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cf
 
 int sweep_pattern(const rkx_method &m) { return pattern_of(m); }
 
-hipError_t launch_sweep(int pattern, int K, const sweep_inst *insts, const int2 *tiles,
+hipError_t launch_sweep(int pattern, int K, bool gen, const sweep_inst *insts, const int2 *tiles,
                         int ntiles, hipStream_t stream)
 {
   if (ntiles <= 0)
     return hipSuccess;
   switch (pattern) {
-  case 1: return launch_pat<pat_dopri54>(K, insts, tiles, ntiles, stream);
-  case 2: return launch_pat<pat_verner65>(K, insts, tiles, ntiles, stream);
-  case 3: return launch_pat<pat_rkf23>(K, insts, tiles, ntiles, stream);
-  case 4: return launch_pat<pat_heun2>(K, insts, tiles, ntiles, stream);
-  case 5: return launch_pat<pat_rk4>(K, insts, tiles, ntiles, stream);
-  case 6: return launch_pat<pat_ssprk3>(K, insts, tiles, ntiles, stream);
+  case 1: return launch_pat<pat_dopri54>(K, gen, insts, tiles, ntiles, stream);
+  case 2: return launch_pat<pat_verner65>(K, gen, insts, tiles, ntiles, stream);
+  case 3: return launch_pat<pat_rkf23>(K, gen, insts, tiles, ntiles, stream);
+  case 4: return launch_pat<pat_heun2>(K, gen, insts, tiles, ntiles, stream);
+  case 5: return launch_pat<pat_rk4>(K, gen, insts, tiles, ntiles, stream);
+  case 6: return launch_pat<pat_ssprk3>(K, gen, insts, tiles, ntiles, stream);
   default: return hipErrorInvalidValue;
   }
 }

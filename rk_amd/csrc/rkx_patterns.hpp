@@ -66,6 +66,9 @@ struct pat_ssprk3 {
   static constexpr bool fsal = false;
 };
 
+// y rows the fused kernel loads ahead of its first stage (bounded by the ring)
+constexpr int prefetch_rows(int s) { return (s - 2 < 2) ? s - 2 : 2; }
+
 // compile-time loop: f(integral_constant<int, I>) for I = 0 .. N-1, so ring
 // slots and stage indices are constants and the register ring stays in VGPRs
 template <class F, int... Is>

@@ -45,6 +45,25 @@ def test_mix_equals_alone_bitwise(rk, ctrl):
             assert np.array_equal(b.result(i), y_alone)
 
 
+@pytest.mark.parametrize("meth,grids", [("DOPRI54", (9, 40, 100)), ("VERNER65", (12, 30, 90))])
+def test_short_and_tall_grids_share_a_launch(rk, meth, grids):
+    """A grid too short for single-reflection ghost rows puts its whole launch
+    group (same table and strip width) on the general reflection form; every
+    instance still equals its alone run bitwise.  VERNER65 also runs whole
+    ring trips, with the requested band height rounded up to fill them."""
+    from rk_amd.workloads import bruss_instance
+    insts = [bruss_instance(N, 2e-3, k, seed=3) for k, N in enumerate(grids)]
+    with rk.Batch(tile_waves=1, tile_rows=24) as b:
+        ids = [b.add(p, meth, 0.0, 0.3, y0, 1e-6, h0=1e-3) for p, y0 in insts]
+        assert b.run(100000) == 0
+        for (p, y0), i in zip(insts, ids):
+            y_alone, st_alone = plain(rk, p, y0, meth, 0.3, 1e-6, 1e-3)
+            st, done = b.status(i)
+            assert done
+            assert (st.steps_acc, st.steps_rej) == (st_alone.steps_acc, st_alone.steps_rej)
+            assert np.array_equal(b.result(i), y_alone)
+
+
 def test_batch_matches_oracle(rk, orc):
     """Device step control with the pow service: each instance reproduces the
     oracle (seq D restated) bitwise."""
