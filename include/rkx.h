@@ -205,7 +205,8 @@ size_t rkx_batch_bytes(const rkx_batch *b);
  * params_dev[4k..4k+3] = (c0, d0, d1, c1), start y0_dev[2k..2k+1], each
  * integrated from t0 to te with its own step size, accept/reject and the
  * stage arithmetic of seq D; the step control uses the device pow (as
- * RKX_CTRL_DEVICE_POW).  h0 > 0 is the initial step of every system.  Writes
+ * RKX_CTRL_DEVICE_POW).  h0 > 0 is the initial step of every system;
+ * 1 <= count <= 2^30.  Writes
  * y_dev[2k..2k+1] and the per-system accepted/rejected counts (device). */
 int rkx_lv_ensemble(int count, const double *params_dev, const double *y0_dev, double *y_dev,
                     double t0, double te, const rkx_method *m, double tol, double h0,

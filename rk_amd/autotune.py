@@ -14,7 +14,7 @@ results), so wall times compare directly.
 """
 import time
 
-CANDIDATES = ((1, 64), (2, 48), (2, 64), (2, 96), (2, 128), (4, 64), (4, 96))
+CANDIDATES = ((1, 64), (2, 64), (2, 96), (2, 128), (2, 160), (2, 192), (4, 64), (4, 128))
 
 
 def batch_tiles(instances, meth="DOPRI54", t0=0.0, te=4.0, tol=1e-6, h0=1e-3, sample=48,

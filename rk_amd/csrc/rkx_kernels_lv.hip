@@ -31,9 +31,10 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_ensemble(int count, lv_method m
                                                           int step_limit, int *__restrict__ acc_out,
                                                           int *__restrict__ rej_out)
 {
-  const int k = blockIdx.x * kLvBlock + threadIdx.x;
-  if (k >= count)
+  const int kt = blockIdx.x * kLvBlock + threadIdx.x;
+  if (kt >= count)
     return;
+  const size_t k = static_cast<size_t>(kt);
   dev_ode p{};
   p.c0 = prm[4 * k + 0];
   p.d0 = prm[4 * k + 1];
@@ -114,6 +115,8 @@ extern "C" int rkx_lv_ensemble(int count, const double *params_dev, const double
 {
   if (count < 1 || !params_dev || !y0_dev || !y_dev || !acc_dev || !rej_dev || h0 <= 0.0)
     return fail(RKX_EINVAL, "rkx_lv_ensemble: bad argument (h0 must be > 0)");
+  if (count > (1 << 30))
+    return fail(RKX_EINVAL, "rkx_lv_ensemble: at most 2^30 systems per call");
   if (int rc = check_method(m))
     return rc;
   lv_method lm{};

@@ -156,3 +156,7 @@ def test_solve_argument_errors_before_any_device_call():
     h = C.c_void_p()
     assert L.rkx_batch_create(C.byref(h), C.byref(bo)) == -1
     assert L.rkx_batch_create(None, C.byref(bo)) == -1
+    ens = lambda count, h0: L.rkx_lv_ensemble(count, dummy, dummy, dummy, 0.0, 1.0, C.byref(m),
+                                              1e-6, h0, 0, dummy, dummy, None)
+    assert ens(0, 1e-3) == -1 and ens(16, 0.0) == -1
+    assert ens((1 << 30) + 1, 1e-3) == -1 and b"2^30" in L.rkx_last_error()
