@@ -36,6 +36,8 @@
 // rejected step costs no copy.  Resident state is two vectors of n doubles.
 #include "rkx_patterns.hpp"
 
+#include <algorithm>
+
 namespace rkx {
 
 namespace {
@@ -402,11 +404,18 @@ int sweep_band_rows(int pattern, int s, int rows)
 // height R for which all strips x bands tiles are resident at once (one
 // dispatch round: a second, partial round would leave most CUs idle while
 // it runs), with bands as tall as that allows (the s-1 halo rows a band
-// recomputes are a smaller share of taller bands).
+// recomputes are a smaller share of taller bands).  Where more than one
+// workgroup fits a CU, every count from the one that gives two waves per
+// SIMD up to the resident maximum is a candidate: on MI355X the FP64 row
+// loop gains little from more than two waves per SIMD (RKF23 at N = 2000:
+// 58 us per step either way with the same bands), so the taller bands of
+// fewer workgroups per CU can win (RKF23: 54.7 us with one per CU against
+// 58.6 us filling both).  The candidate with the lowest modelled cost wins.
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
 {
   const int s = m.s;
   const int N = p.N;
+  const int pat = pattern_of(m);
   int cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -414,8 +423,6 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     (void) hipGetLastError();
     cus = 256;
   }
-  // for each strip width, the band height that fills one dispatch round;
-  // then the width with the lowest modelled cost
   sweep_cfg best{2, 4};
   double best_cost = 1e300;
   for (int K = 1; K <= 4; ++K) {
@@ -423,19 +430,21 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     if (oc < 1)
       continue;
     const int nstrips = (N + oc - 1) / oc;
-    const int slots = cus * resident_blocks(pattern_of(m), K);
-    int bands = slots / nstrips;
-    if (bands < 1)
-      bands = 1;
-    int rows = (N + bands - 1) / bands;
-    if (rows < 4)
-      rows = 4;
-    rows = sweep_band_rows(pattern_of(m), s, rows);
-    const double cost = sweep_tile_cost(N, s, K, rows, nullptr,
-                                        full_trips_of(pattern_of(m)) ? s + 1 : 0);
-    if (cost < best_cost) {
-      best_cost = cost;
-      best = sweep_cfg{K, rows};
+    const int res = resident_blocks(pat, K);
+    const int per_cu_min = std::min(res, (4 + K - 1) / K);   // 2K waves each: 2 per SIMD
+    for (int per_cu = per_cu_min; per_cu <= res; ++per_cu) {
+      int bands = cus * per_cu / nstrips;
+      if (bands < 1)
+        bands = 1;
+      int rows = (N + bands - 1) / bands;
+      if (rows < 4)
+        rows = 4;
+      rows = sweep_band_rows(pat, s, rows);
+      const double cost = sweep_tile_cost(N, s, K, rows, nullptr, full_trips_of(pat) ? s + 1 : 0);
+      if (cost < best_cost) {
+        best_cost = cost;
+        best = sweep_cfg{K, rows};
+      }
     }
   }
   return best;
