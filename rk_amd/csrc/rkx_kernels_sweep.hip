@@ -17,7 +17,9 @@
 //
 // Neighbours.  Vertical neighbours are the lane's own registers.  Horizontal
 // neighbours and the other component of the same grid point are exchanged
-// through LDS once per row iteration (double buffered, one barrier).  Lanes
+// through LDS once per row iteration (double buffered, one barrier); each
+// value is written to the next iteration's buffer as soon as it is final, so
+// the barrier does not wait for a burst of writes.  Lanes
 // at the strip edges and rows at the band edges are halo: each stage's valid
 // region shrinks by one grid point per stage, so a strip produces L - 2s
 // output columns and a band recomputes s-1 rows on each side (the
@@ -170,6 +172,12 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   // waves (VAR), so that each is straight-line code per row iteration.
   auto rows = [&](auto vt) {
     constexpr int VAR = decltype(vt)::value;
+    // the first iteration's exchange: the band's first y row for stage 0,
+    // zero accumulators of the rows above the band for the other stages
+    static_for<s>([&](auto itg) {
+      constexpr int i = decltype(itg)::value;
+      xch[0][i][VAR][cl] = (i == 0) ? yr[0] : 0.0;
+    });
     for (int rb = it0; rb < it1; rb += NS) {
       static_for<NS>([&](auto ut) {
         constexpr int U = decltype(ut)::value;
@@ -194,10 +202,6 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
         // full trips, iterations past it1 work on rows below the band, which
         // are never stored either.
         const int buf = (rr - it0) & 1;
-        static_for<s>([&](auto itg) {
-          constexpr int i = decltype(itg)::value;
-          xch[buf][i][VAR][cl] = (i == 0) ? yr[U % NS] : acc[(U - i + NS) % NS][i];
-        });
         __syncthreads();
         double nX[s], nY[s], nP[s];
         static_for<s>([&](auto itg) {
@@ -206,6 +210,11 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
           nY[i] = xch[buf][i][VAR][slotY];
           nP[i] = xch[buf][i][VAR ^ 1][cl];           // the other component
         });
+        // The next iteration's exchange goes to the other buffer as each
+        // value becomes final: its stage-0 row now, w_{i+1} of row rr - i
+        // right after stage i's scatter.  Every wave read that buffer in the
+        // previous iteration, before the barrier above.
+        xch[buf ^ 1][0][VAR][cl] = yr[(U + 1) % NS];
 
         static_for<s>([&](auto itg) {
           constexpr int i = decltype(itg)::value;
@@ -254,6 +263,8 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
             if constexpr ((P::e >> i) & 1u)
               acc[sl][s] += kc[s - i] * f;
           }
+          if constexpr (i + 1 < s)
+            xch[buf ^ 1][i + 1][VAR][cl] = acc[sl][i + 1];
           if constexpr (i == s - 1) {
             // y_new = y + dy; the error term against y_old (block-inline.h:298-320)
             const double ynw = yold + acc[sl][0];
