@@ -124,6 +124,10 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   double yr[NS];                                 // y rows of stage 0, ring by row
   double yo[NS];                                 // y_old of finishing rows, ring by iteration
   constexpr int DY = 2;                          // y_old loaded DY iterations ahead
+  // the first stored row (kl = r0) finishes at iteration it0 + 2(s-1); its
+  // y_old must have been loaded by then, i.e. in an iteration >= it0
+  static_assert(DY <= 2 * (s - 1) && DY + 1 <= NS, "y_old prefetch outruns the band");
+  static_assert(PF + 3 <= NS, "y ring too small for the prefetch");
   gconst *yc = y + 2 * ccol + var;
   // Ghost rows: a single reflection about row 0 / N-1, min(|k|, 2(N-1) - |k|),
   // covers every row a band loads once the grid is taller than the halo plus
