@@ -121,6 +121,12 @@ typedef struct rkx_opts {
   double *trace;         /* optional host buffer, 3 doubles (t, h, err) per attempt */
   int trace_cap;         /* attempts the trace buffer holds */
   void *stream;          /* hipStream_t to run on; NULL: the library's own stream */
+  const char *dump_path; /* optional: every attempted step appended to this file in the
+                            reference's DUMP format (include/tools.h:239-256): t, h and
+                            the step's new y before the accept/reject decision, "%.20e"
+                            tab-separated, one line per attempt; the file is created at
+                            the first attempt.  Implies RKX_CTRL_HOST (bitwise the same
+                            steps) and one device-to-host copy of y per attempt. */
 } rkx_opts;
 
 typedef struct rkx_stats {

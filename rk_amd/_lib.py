@@ -37,7 +37,7 @@ class Opts(C.Structure):
     _fields_ = [("h0", C.c_double), ("step_limit", C.c_int), ("impl", C.c_int),
                 ("ctrl", C.c_int), ("tile_rows", C.c_int), ("tile_waves", C.c_int),
                 ("trace", C.POINTER(C.c_double)), ("trace_cap", C.c_int),
-                ("stream", C.c_void_p)]
+                ("stream", C.c_void_p), ("dump_path", C.c_char_p)]
 
 
 class Stats(C.Structure):

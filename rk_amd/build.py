@@ -40,6 +40,11 @@ DROPIN = {
                                          "RKX_DROPIN_IMPL=RKX_IMPL_D"),
     "rk_lv_dopri54": ("lv", "LV_COUNT=10000 T_END=4.0 TOL=1E-6 H_INIT=1E-3 METHOD=DOPRI54 "
                             "STEP_LIMIT=0"),
+    # -DDUMP: the per-attempt trajectory file of include/tools.h:239-256
+    "rk_bruss2d_dump": ("bruss2d", "BRUSS_GRID_SIZE=4 T_END=4.0 TOL=1E-6 H_INIT=1E-3 "
+                                   "METHOD=DOPRI54 STEP_LIMIT=0 DUMP"),
+    "rk_lv_dump": ("lv", "LV_COUNT=3 T_END=20.0 TOL=1E-8 H_INIT=1E-3 METHOD=DOPRI54 "
+                         "STEP_LIMIT=0 DUMP"),
 }
 BIN = os.path.join(HERE, "bin")
 

@@ -62,6 +62,9 @@ void solver(double t0, double te, double *y0, double *y, double tol)
   o.h0 = h;
   o.step_limit = STEP_LIMIT;
   o.impl = RKX_DROPIN_IMPL;
+#ifdef DUMP
+  o.dump_path = "DUMP"; /* every attempted step, as step_control does (tools.h:239-256) */
+#endif
   rkx_stats st;
   if (rkx_solve_host(&p, &m, t0, te, y0, y, tol, &o, &st) != RKX_OK) {
     fprintf(stderr, "rkx: %s\n", rkx_last_error());

@@ -256,6 +256,30 @@ void host_finish(state &st, bool decided, const dev_method &dm, dev_coef &hc)
   premult(dm, st.s, st.h, hc);
 }
 
+// DUMP output (include/tools.h:239-256): one line per attempted step, t and
+// h then every component of the step's new y, "%.20e" separated by tabs.  The
+// reference creates the file at the first attempt and appends afterwards.
+struct dump_file {
+  FILE *f = nullptr;
+  ~dump_file()
+  {
+    if (f)
+      std::fclose(f);
+  }
+  int append(const char *path, double t, double h, const std::vector<double> &y)
+  {
+    if (!f && !(f = std::fopen(path, "w")))
+      return fail(RKX_EINVAL, std::string("rkx_solve: cannot open dump file ") + path);
+    std::fprintf(f, "%.20e\t%.20e", t, h);
+    for (double v : y)
+      std::fprintf(f, "\t%.20e", v);
+    std::fputc('\n', f);
+    if (std::ferror(f))
+      return fail(RKX_EINVAL, std::string("rkx_solve: write to dump file failed: ") + path);
+    return RKX_OK;
+  }
+};
+
 }  // namespace rkx
 
 using namespace rkx;
@@ -467,6 +491,8 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     return fail(RKX_EINVAL, "rkx_solve: unknown step-control mode");
   if (o.trace_cap < 0 || (o.trace_cap > 0 && !o.trace))
     return fail(RKX_EINVAL, "rkx_solve: trace_cap without a trace buffer");
+  if (o.dump_path)
+    o.ctrl = RKX_CTRL_HOST;          // the dump needs every attempted step on the host
   hipStream_t stream = static_cast<hipStream_t>(o.stream);
   if (!stream)
     if (int rc = default_stream(&stream))
@@ -600,12 +626,21 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     // One host round trip per step: the device computes the error norm, the
     // host runs the reference's step control with the C library's pow.
     state cur = hs;
+    dump_file dump;
+    std::vector<double> ydump;
     while (!cur.done) {
       RKX_HIP(enqueue_step());
       RKX_HIP(hipMemcpyAsync(hs_pinned, st_dev, sizeof(state), hipMemcpyDeviceToHost, stream));
       RKX_HIP(hipStreamSynchronize(stream));
       cur = hs_pinned[0];
       const int attempt = cur.acc + cur.rej;
+      if (o.dump_path) {
+        // the step's new y is in the other buffer until the decision flips cur
+        ydump.resize(n);
+        RKX_HIP(hipMemcpy(ydump.data(), ybuf[cur.cur ^ 1], vec, hipMemcpyDeviceToHost));
+        if (int rc = dump.append(o.dump_path, cur.t, cur.h, ydump))
+          return rc;
+      }
       if (o.trace && attempt < o.trace_cap) {
         o.trace[3 * attempt + 0] = cur.t;
         o.trace[3 * attempt + 1] = cur.h;

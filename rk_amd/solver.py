@@ -5,6 +5,7 @@ the state stays resident in HBM; `solver` keeps the reference's signature
 solver(t0, te, y0, y, tol) on host numpy arrays.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -27,7 +28,7 @@ def _ptr(x):
 
 
 def _opts(impl="PipeDls", ctrl="device", h0=0.0, step_limit=0, tile_rows=0, tile_waves=0,
-          trace_cap=0, stream=None):
+          trace_cap=0, stream=None, dump_path=None):
     o = Opts()
     lib().rkx_opts_default(C.byref(o))
     o.impl = IMPLS[impl] if isinstance(impl, str) else impl
@@ -42,6 +43,8 @@ def _opts(impl="PipeDls", ctrl="device", h0=0.0, step_limit=0, tile_rows=0, tile
         o.trace = trace.ctypes.data_as(C.POINTER(C.c_double))
         o.trace_cap = trace_cap
     o.stream = stream or 0
+    if dump_path is not None:
+        o.dump_path = os.fsencode(dump_path)
     return o, trace
 
 

@@ -160,3 +160,32 @@ def test_solve_argument_errors_before_any_device_call():
                                               1e-6, h0, 0, dummy, dummy, None)
     assert ens(0, 1e-3) == -1 and ens(16, 0.0) == -1
     assert ens((1 << 30) + 1, 1e-3) == -1 and b"2^30" in L.rkx_last_error()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the header's structs (rk_amd/_lib.py, rk_amd/batch.py)
+    have the C compiler's size and field offsets, so no field is read at the
+    wrong place when the header changes."""
+    import ctypes as C
+    import subprocess
+    from rk_amd._lib import Method, Ode, Opts, Stats
+    from rk_amd.batch import BatchOpts
+    structs = {"rkx_ode": Ode, "rkx_method": Method, "rkx_opts": Opts, "rkx_stats": Stats,
+               "rkx_batch_opts": BatchOpts}
+    lines = ['#include "rkx.h"', "#include <stddef.h>", "#include <stdio.h>", "int main(void) {"]
+    want = []
+    for cname, cls in structs.items():
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        want.append(C.sizeof(cls))
+        for fname, _ in cls._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, fname))
+            want.append(getattr(cls, fname).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(REPO, "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == want

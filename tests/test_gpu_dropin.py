@@ -50,3 +50,25 @@ def test_dropin_binary_matches_reference(binary, case):
     m = re.search(r"^Result: y\[0\] = (\S+), \.\.\., y\[\d+\] = (\S+); min = (\S+), max = (\S+)$",
                   r.stdout, re.M)
     assert list(m.groups()) == c["footprint"]
+
+
+DUMP_CASES = [("rk_bruss2d_dump", "dump_bruss_n4_dopri54"), ("rk_lv_dump", "dump_lv_n3_dopri54")]
+
+
+@pytest.mark.parametrize("binary,case", DUMP_CASES)
+def test_dropin_dump_file_matches_reference(binary, case, tmp_path):
+    """Built with -DDUMP, the binary writes ./DUMP as the reference's
+    step_control does (include/tools.h:239-256): every attempted step's t, h
+    and new y, equal to the reference's file value for value."""
+    import numpy as np
+    path = os.path.join(BIN, binary)
+    assert os.path.exists(path), "drop-in binaries are built by rk_amd/build.py"
+    print("running", binary, flush=True)
+    r = subprocess.run([path], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    z = np.load(os.path.join(HERE, "golden", case + ".npz"))
+    rows = [[float(x) for x in ln.split("\t")] for ln in open(tmp_path / "DUMP")]
+    got = np.array(rows)
+    assert got.shape == (len(z["t"]), 2 + z["Y"].shape[1])
+    assert np.array_equal(got[:, 0], z["t"]) and np.array_equal(got[:, 1], z["h"])
+    assert np.array_equal(got[:, 2:], z["Y"])

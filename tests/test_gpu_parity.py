@@ -138,6 +138,30 @@ def test_golden_dump_host_control(rk, name, impl):
     assert np.array_equal(y, z["Y"][-1])
 
 
+@pytest.mark.parametrize("name", sorted(DUMPS))
+@pytest.mark.parametrize("impl", ["D", "PipeDls"])
+def test_golden_dump_file(rk, name, impl, tmp_path):
+    """rkx_opts.dump_path writes the reference's DUMP file (include/tools.h:239-256):
+    every attempted step's t, h and new y (rejected steps included) equal the
+    reference's, value for value."""
+    meta = DUMPS[name]
+    d = meta["defines"]
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    prob = "bruss2d" if meta["problem"].startswith("bruss") else "lv"
+    grid = int(d.get("BRUSS_GRID_SIZE", d.get("LV_COUNT", 0)))
+    p = gpu_problem(rk, prob, grid)
+    m = rk.method(d["METHOD"])
+    tol = float(d["TOL"]) if m.has_emb else 0.0
+    path = tmp_path / "DUMP"
+    y, st, _ = run_gpu(rk, p, m, 0.0, float(d["T_END"]), p.ode_start(), tol,
+                       h0=float(d["H_INIT"]), impl=impl, dump_path=str(path))
+    rows = np.array([[float(x) for x in ln.split("\t")] for ln in open(path)])
+    assert rows.shape == (len(z["t"]), 2 + z["Y"].shape[1])
+    assert np.array_equal(rows[:, 0], z["t"]) and np.array_equal(rows[:, 1], z["h"])
+    assert np.array_equal(rows[:, 2:], z["Y"])
+    assert (st.steps_acc, st.steps_rej) == (meta["acc"], meta["rej"])
+
+
 CHAIN = [(3, "DOPRI54"), (7, "RKF23"), (1010, "DOPRI54"), (1011, "DOPRI54"),
          (2500, "VERNER65"), (4100, "RK4"), (3000, "SSPRK3"), (999, "HEUN2")]
 
