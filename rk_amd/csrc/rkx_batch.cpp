@@ -92,16 +92,23 @@ void free_inst(batch_inst &b)
 // per component for grids that fit a 64-column strip, else two (measured
 // fastest per lane when many instances share the CUs), and bands of 64 rows
 // (long enough for the s-1 halo rows to be a small share, short enough that
-// large instances split into many tiles).  The options override either.
+// large instances split into many tiles).  The options override either; a
+// strip width wider than the table's kernel is built for (DOPRI87: 2 waves)
+// is narrowed to the widest it has.
 sweep_cfg batch_cfg(const dev_ode &p, const rkx_method &m, const rkx_batch_opts &o)
 {
   sweep_cfg c;
   c.waves = (p.N + 2 * m.s <= 64) ? 1 : 2;
   c.rows = 64;
   if (o.tile_waves > 0)
-    c.waves = o.tile_waves;
+    c.waves = std::min(o.tile_waves, 4);
   if (o.tile_rows > 0)
     c.rows = o.tile_rows;
+  const int pattern = sweep_pattern(m);
+  while (c.waves > 1 && sweep_resident_blocks(pattern, c.waves) < 1)
+    --c.waves;
+  while (c.waves < 4 && 64 * c.waves <= 2 * m.s)
+    ++c.waves;
   return c;
 }
 

@@ -584,6 +584,9 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   chain_inst *chain_dev = nullptr;
   int ntiles = 0;
   const int pattern = sweep_pattern(*m);
+  if (impl != RKX_IMPL_D && !chain && sweep_resident_blocks(pattern, cfg.waves) < 1)
+    return fail(RKX_EINVAL, "rkx_solve: tile_waves wider than this table's fused kernel "
+                            "(DOPRI87: at most 2)");
   if (chain) {
     if (int rc = mem.alloc(reinterpret_cast<void **>(&chain_dev), sizeof(chain_inst)))
       return rc;

@@ -235,6 +235,13 @@ sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cf
 // general reflection form (some instance fails sweep_single_reflect)
 hipError_t launch_sweep(int pattern, int K, bool gen, const sweep_inst *insts, const int2 *tiles,
                         int ntiles, hipStream_t stream);
+// the DOPRI87 kernels (their own translation unit: 13 stages take long to build)
+hipError_t launch_sweep_dopri87(int K, bool gen, const sweep_inst *insts, const int2 *tiles,
+                                int ntiles, hipStream_t stream);
+int resident_blocks_dopri87(int K);
+// workgroups of the fused kernel resident per CU for a pattern and strip
+// width; 0: the strip is too wide for that table's kernel
+int sweep_resident_blocks(int pattern, int K);
 
 // One LV chain instance as the fused chain kernel sees it (device memory).
 struct chain_inst {

@@ -165,7 +165,8 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
 
 bool chain_supported(const dev_ode &p, const rkx_method &m)
 {
-  return p.kind == RKX_LV_CHAIN && pattern_of(m) != 0 && kT > 2 * m.s;
+  const int pat = pattern_of(m);
+  return p.kind == RKX_LV_CHAIN && pat >= 1 && pat <= 6 && kT > 2 * m.s;
 }
 
 chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, const dev_coef *cf,

@@ -57,6 +57,18 @@ struct pat_rk4 {
   static constexpr unsigned b = 0xF, e = 0xF;
   static constexpr bool fsal = false;
 };
+// DOPRI87 (13 stages): its register ring needs 256 VGPRs and about 130 AGPRs,
+// one wave per SIMD, so strips of at most 2 waves per component; whole trips
+// (the exit test inside a trip would cost more registers still).
+struct pat_dopri87 {
+  static constexpr bool full_trips = true;
+  static constexpr int occ = 1;
+  static constexpr int s = 13;
+  static constexpr unsigned rowA[13] = {0x0,  0x1,  0x3,   0x5,   0xD,   0x19,  0x39,
+                                        0x79, 0xF9, 0x1F9, 0x3F9, 0x7F9, 0x7F9};
+  static constexpr unsigned b = 0x1FE1, e = 0x1FE1;
+  static constexpr bool fsal = false;
+};
 struct pat_ssprk3 {
   static constexpr bool full_trips = false;
   static constexpr int occ = 2;
@@ -95,7 +107,7 @@ __device__ __forceinline__ constexpr bool nzA(int l, int i)
 // block-inline.h:324-428), which leaves every sum bitwise unchanged.
 inline int pattern_of(const rkx_method &m)
 {
-  // 1 DOPRI54, 2 VERNER65, 3 RKF23, 4 HEUN2, 5 RK4, 6 SSPRK3; 0 none
+  // 1 DOPRI54, 2 VERNER65, 3 RKF23, 4 HEUN2, 5 RK4, 6 SSPRK3, 7 DOPRI87; 0 none
   auto rowmask = [&](int l) {
     unsigned r = 0;
     for (int i = 0; i < l; ++i)
@@ -127,11 +139,13 @@ inline int pattern_of(const rkx_method &m)
   if (match(2, pat_heun2::rowA, pat_heun2::b, pat_heun2::e)) return 4;
   if (match(4, pat_rk4::rowA, pat_rk4::b, pat_rk4::e)) return 5;
   if (match(3, pat_ssprk3::rowA, pat_ssprk3::b, pat_ssprk3::e)) return 6;
+  if (match(13, pat_dopri87::rowA, pat_dopri87::b, pat_dopri87::e)) return 7;
   return 0;
 }
 
 
-// Calls f(pat_X{}) for the compiled pattern id (1..6).
+// Calls f(pat_X{}) for the compiled pattern id 1..6 (the tables the LV chain
+// kernel is built for; see chain_supported).
 template <class F>
 inline hipError_t dispatch_pattern(int pattern, F &&f)
 {

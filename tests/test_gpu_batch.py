@@ -64,6 +64,24 @@ def test_short_and_tall_grids_share_a_launch(rk, meth, grids):
             assert np.array_equal(b.result(i), y_alone)
 
 
+def test_dopri87_instances_in_a_batch(rk):
+    """DOPRI87 instances run their fused kernel in a batch (strip width
+    narrowed to the 2 waves it is built for when the options ask for 4) and
+    equal their alone runs bitwise."""
+    from rk_amd.workloads import bruss_instance
+    insts = [bruss_instance(N, a, k, seed=4) for k, (N, a) in enumerate([(24, 2e-3), (70, 3e-3),
+                                                                          (150, 1.5e-3)])]
+    with rk.Batch(tile_waves=4, tile_rows=40) as b:
+        ids = [b.add(p, "DOPRI87", 0.0, 0.3, y0, 1e-8, h0=1e-4) for p, y0 in insts]
+        assert b.run(100000) == 0
+        for (p, y0), i in zip(insts, ids):
+            y_alone, st_alone = plain(rk, p, y0, "DOPRI87", 0.3, 1e-8, 1e-4)
+            st, done = b.status(i)
+            assert done and st.resident_bytes == 2 * p.n * 8
+            assert (st.steps_acc, st.steps_rej) == (st_alone.steps_acc, st_alone.steps_rej)
+            assert np.array_equal(b.result(i), y_alone)
+
+
 def test_batch_matches_oracle(rk, orc):
     """Device step control with the pow service: each instance reproduces the
     oracle (seq D restated) bitwise."""

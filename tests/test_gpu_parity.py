@@ -162,6 +162,31 @@ def test_golden_dump_file(rk, name, impl, tmp_path):
     assert (st.steps_acc, st.steps_rej) == (meta["acc"], meta["rej"])
 
 
+@pytest.mark.parametrize("N,waves,rows", [(20, 0, 0), (40, 1, 9), (40, 2, 40), (130, 0, 0),
+                                          (130, 2, 17)])
+def test_dopri87_fused_equals_d_form(rk, N, waves, rows):
+    """DOPRI87 (13 stages) through its fused kernel (one wave per SIMD, whole
+    ring trips, band heights rounded up to fill them) gives the D form's
+    steps and vector bitwise, for short grids (general ghost rows), both
+    strip widths and several band heights; two vectors resident."""
+    p = rk.Bruss2D(N, 3e-3)
+    y0 = p.ode_start()
+    yd, std, _ = run_gpu(rk, p, "DOPRI87", 0.0, 0.5, y0, 1e-9, h0=1e-4, impl="D")
+    yf, stf, _ = run_gpu(rk, p, "DOPRI87", 0.0, 0.5, y0, 1e-9, h0=1e-4, impl="PipeDls",
+                         tile_waves=waves, tile_rows=rows)
+    assert (stf.steps_acc, stf.steps_rej) == (std.steps_acc, std.steps_rej)
+    assert np.array_equal(yf, yd)
+    assert stf.resident_bytes < std.resident_bytes
+    assert stf.resident_bytes <= 2 * p.n * 8 + 65536
+
+
+def test_dopri87_strip_too_wide_is_an_error(rk):
+    p = rk.Bruss2D(300)
+    with pytest.raises(rk.RkxError):
+        run_gpu(rk, p, "DOPRI87", 0.0, 0.01, p.ode_start(), 1e-9, h0=1e-4, impl="PipeDls",
+                tile_waves=4)
+
+
 CHAIN = [(3, "DOPRI54"), (7, "RKF23"), (1010, "DOPRI54"), (1011, "DOPRI54"),
          (2500, "VERNER65"), (4100, "RK4"), (3000, "SSPRK3"), (999, "HEUN2")]
 
