@@ -83,11 +83,11 @@ def d_form_passes(s):
 
 
 def c2_matrix_leg(window=(300, 1300)):
-    """BRUSS2D N=2000, t in [0, 1], h0 1e-3: {DOPRI54, VERNER65} x tol {1e-6,
-    1e-8} x {full storage D form, low-storage fused PipeDls}; each timed over
-    attempted steps window[0]..window[1] of the solve (differenced, so set-up
-    is excluded), with the algorithmic HBM bytes of the form and the device
-    bytes the solve holds."""
+    """BRUSS2D N=2000, t in [0, 1], h0 1e-3: {DOPRI54, VERNER65, and the 13-stage
+    DOPRI87} x tol {1e-6, 1e-8} x {full storage D form, low-storage fused
+    PipeDls}; each timed over attempted steps window[0]..window[1] of the
+    solve (differenced, so set-up is excluded), with the algorithmic HBM bytes
+    of the form and the device bytes the solve holds."""
     import torch
     import rk_amd
     p = rk_amd.Bruss2D(N_GRID, 2e-3)
@@ -95,7 +95,7 @@ def c2_matrix_leg(window=(300, 1300)):
     y = torch.empty_like(y0)
     torch.cuda.synchronize()
     rows = []
-    for meth in ("DOPRI54", "VERNER65"):
+    for meth in ("DOPRI54", "VERNER65", "DOPRI87"):
         s = rk_amd.method(meth).s
         for tol in (1e-6, 1e-8):
             for impl in ("D", "PipeDls"):
