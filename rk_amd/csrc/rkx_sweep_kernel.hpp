@@ -45,6 +45,10 @@
 
 namespace rkx {
 
+// Internal linkage on purpose: each translation unit that includes this
+// header instantiates the kernels of its own tables (rkx_kernels_sweep.hip:
+// DOPRI54 ... SSPRK3; rkx_kernels_sweep_dopri87.hip: DOPRI87) and exports
+// only the launch and occupancy functions declared in rkx_internal.hpp.
 namespace {
 
 // even periodic extension of a row/column index about 0 and N-1 (ghost
