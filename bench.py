@@ -74,6 +74,32 @@ def cpu_baseline(threads):
                       "attempted steps"}
 
 
+def cpu_single_thread():
+    """The reference's seq D (one thread) on the first 20 attempted steps of
+    the same workload, when that binary was built (oracle/_ref)."""
+    ref = os.path.join(REPO, "oracle", "_ref", "seq_bruss_D_n2000_dopri54_sl20")
+    if not os.path.exists(ref):
+        return None
+    out = subprocess.run([ref], capture_output=True, text=True, timeout=600, cwd="/tmp").stdout
+    m = re.search(r"Kernel time: (\S+) s \((\S+) s per step\)", out)
+    steps = re.search(r"Number of steps: (\d+)", out)
+    return {"value": 1.0 / float(m.group(2)), "unit": "steps/s", "cores": 1, "kind": "reference",
+            "sample": "seq D (reference C sources), BRUSS2D N=2000 DOPRI54 tol 1e-6 h0 1e-3, "
+                      "first %s attempted steps (STEP_LIMIT=20)" % steps.group(1)}
+
+
+def host_info():
+    """CPU model and logical CPU count of the host the CPU baselines ran on."""
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        m = re.search(r"^Model name:\s*(.+)$", out, re.M)
+        model = m.group(1).strip() if m else ""
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count()}
+
+
 def d_form_passes(s):
     """Vector passes (reads + writes of n doubles) of one step of the stage
     kernels of the D form: first stage reads y and writes w_1..w_{s-1}, dy,
@@ -335,6 +361,8 @@ def main():
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(threads)
+        res["cpu_single_thread"] = cpu_single_thread()
+        res["host"] = host_info()
     if not a.no_batch and a.step_limit == 0:
         res["c2_matrix"] = c2_matrix_leg()
         print("c2 matrix done", file=sys.stderr, flush=True)
