@@ -310,6 +310,28 @@ def test_zero_length_interval(rk):
     assert st.steps == 0 and np.array_equal(y, y0)
 
 
+def test_large_grid_fused_equals_d_form(rk):
+    """A 20000 x 20000 grid (n = 8e8 doubles, 6.4 GB per vector; the D form
+    holds 64 GB): the first attempted steps through the fused kernel and the
+    D form agree bitwise -- row offsets and tile counts near the index limits
+    of the two forms' different addressing."""
+    import torch
+    N = 20000
+    p = rk.Bruss2D(N, 2e-3)
+    print("N=%d: building the start vector" % N, flush=True)
+    y0 = dev(p.ode_start())
+    out = {}
+    for impl in ("PipeDls", "D"):
+        print("N=%d: %s" % (N, impl), flush=True)
+        y = torch.empty_like(y0)
+        st, _ = rk.solve(p, "DOPRI54", 0.0, 1.0, y0, y, 1e-6, h0=1e-7, step_limit=3, impl=impl)
+        torch.cuda.synchronize()
+        out[impl] = (y, (st.steps_acc, st.steps_rej))
+    assert out["PipeDls"][1] == out["D"][1] and sum(out["D"][1]) == 3
+    assert torch.equal(out["PipeDls"][0], out["D"][0])
+    assert not torch.equal(out["D"][0], y0)                # the solution moved
+
+
 def test_step_limit(rk):
     p = rk.Bruss2D(40)
     _, st, _ = run_gpu(rk, p, "DOPRI54", 0.0, 4.0, p.ode_start(), 1e-6, h0=1e-3, step_limit=37)
