@@ -215,12 +215,6 @@ struct sweep_inst {
 bool sweep_supported(const dev_ode &p, const rkx_method &m);
 int sweep_pattern(const rkx_method &m);   // compiled zero pattern (0: none)
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m);
-// Tile-shape cost model: relative cost of one lane x row iteration for strips
-// of K = 1..4 wavefronts per component (f[K-1]); f == nullptr or f[K-1] <= 0
-// selects the measured defaults.  Returns the modelled cost of one step of an
-// N x N instance with strips of 64K columns and bands of R rows; trip > 0
-// rounds a band's row iterations up to whole trips of that many rows.
-double sweep_tile_cost(int N, int s, int K, int R, const double *f, int trip);
 // Band height a tile gets for `rows` requested rows: unchanged, or for tables
 // whose fused kernel runs whole trips of its s+1-row ring, rounded up so that
 // the band's rows + 2(s-1) row iterations fill whole trips.

@@ -36,23 +36,30 @@ bool sweep_supported(const dev_ode &p, const rkx_method &m)
 // (6 per workgroup) do not pack two workgroups per CU.
 constexpr double kTileCost[4] = {1.36, 1.0, 1.67, 0.97};
 
-double sweep_tile_cost(int N, int s, int K, int R, const double *f, int trip)
+namespace {
+
+// Relative cost of one step of an N x N instance with strips of 64K columns
+// and bands of R rows: lanes x row iterations (the balanced band height plus
+// the 2(s-1) halo iterations and a prologue of about 4) x the measured
+// per-lane factor of the strip width.  trip > 0 rounds a band's row
+// iterations up to whole trips of that many rows.
+double tile_work(int N, int s, int K, int R, int trip)
 {
   const int L = 64 * K;
   const int oc = L - 2 * s;
   if (oc < 1 || R < 1 || N < 1)
     return 1e300;
-  const double fk = (f && f[K - 1] > 0.0) ? f[K - 1] : kTileCost[K - 1];
   const long long nstrips = (N + oc - 1) / oc;
   const long long nbands = (N + R - 1) / R;
   const long long reff = (N + nbands - 1) / nbands;
   long long iters = reff + 2 * (s - 1);
   if (trip > 0)
     iters = (iters + trip - 1) / trip * trip;
-  // + 4: the prologue of a tile (coefficients, first rows) in row iterations
-  return fk * static_cast<double>(nstrips * L) * static_cast<double>(nbands) *
+  return kTileCost[K - 1] * static_cast<double>(nstrips * L) * static_cast<double>(nbands) *
          static_cast<double>(iters + 4);
 }
+
+}  // namespace
 
 namespace {
 
@@ -97,11 +104,17 @@ int sweep_band_rows(int pattern, int s, int rows)
 // 58 us per step either way with the same bands), so the taller bands of
 // fewer workgroups per CU can win (RKF23: 54.7 us with one per CU against
 // 58.6 us filling both).  The candidate with the lowest modelled cost wins.
+// A grid so wide that one band per strip gives fewer tiles than CUs (N =
+// 32767, K = 4: 136 strips) or more than fit at once (DOPRI87, N = 32767:
+// 322 strips of 2 waves) instead takes shorter bands over several rounds
+// when that shortens the step: a CU then works through its tiles in turn,
+// ceil(tiles / CUs) x the row iterations of one.
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
 {
   const int s = m.s;
   const int N = p.N;
   const int pat = pattern_of(m);
+  const int trip = full_trips_of(pat) ? s + 1 : 0;
   int cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -109,6 +122,12 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     (void) hipGetLastError();
     cus = 256;
   }
+  auto band_rows_for = [&](long long bands) {
+    if (bands < 1)
+      bands = 1;
+    int rows = static_cast<int>((N + bands - 1) / bands);
+    return sweep_band_rows(pat, s, rows < 4 ? 4 : rows);
+  };
   sweep_cfg best{2, 4};
   double best_cost = 1e300;
   for (int K = 1; K <= 4; ++K) {
@@ -121,17 +140,33 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
       continue;                                     // too wide for this table's kernel
     const int per_cu_min = std::min(res, (4 + K - 1) / K);   // 2K waves each: 2 per SIMD
     for (int per_cu = per_cu_min; per_cu <= res; ++per_cu) {
-      int bands = cus * per_cu / nstrips;
-      if (bands < 1)
-        bands = 1;
-      int rows = (N + bands - 1) / bands;
-      if (rows < 4)
-        rows = 4;
-      rows = sweep_band_rows(pat, s, rows);
-      const double cost = sweep_tile_cost(N, s, K, rows, nullptr, full_trips_of(pat) ? s + 1 : 0);
+      const int rows = band_rows_for(static_cast<long long>(cus) * per_cu / nstrips);
+      const double cost = tile_work(N, s, K, rows, trip);
       if (cost < best_cost) {
         best_cost = cost;
         best = sweep_cfg{K, rows};
+      }
+    }
+  }
+  // several rounds for grids too wide for one band per strip to fill the CUs
+  const int oc = 64 * best.waves - 2 * s;
+  const long long nstrips = (N + oc - 1) / oc;
+  auto round_time = [&](int rows) {
+    const long long tiles = nstrips * ((N + rows - 1) / rows);
+    long long iters = rows + 2 * (s - 1);
+    if (trip > 0)
+      iters = (iters + trip - 1) / trip * trip;
+    return static_cast<double>((tiles + cus - 1) / cus) * static_cast<double>(iters + 4);
+  };
+  const long long tiles = nstrips * ((N + best.rows - 1) / best.rows);
+  if (tiles < cus || tiles > static_cast<long long>(cus) * resident_blocks(pat, best.waves)) {
+    double best_time = round_time(best.rows);
+    for (int k = 2; k <= 16; ++k) {
+      const int rows = band_rows_for(static_cast<long long>(k) * cus / nstrips);
+      const double t = round_time(rows);
+      if (t < best_time) {
+        best_time = t;
+        best.rows = rows;
       }
     }
   }
