@@ -276,6 +276,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--step-limit", type=int, default=0,
                     help="cap each solve at this many attempted steps (profiling runs)")
+    ap.add_argument("--leg", choices=("c2_matrix", "batched", "ensemble", "serving"),
+                    help="run only this leg and print its JSON (for profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 5 plain device copies of n doubles, the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -285,6 +287,12 @@ def main():
               "grid would shard by row bands with one halo exchange and one max all-reduce "
               "per step; batched instances would shard whole); use --gpus 1", file=sys.stderr)
         return 2
+
+    if a.leg:
+        legs = {"c2_matrix": c2_matrix_leg, "batched": batched_leg, "ensemble": ensemble_leg,
+                "serving": lambda: serving_leg(a.serve_rate, a.serve_seconds)}
+        print(json.dumps({"leg": a.leg, "result": legs[a.leg]()}))
+        return 0
 
     import numpy as np
     import torch
