@@ -99,15 +99,13 @@ enum {
 };
 
 enum {
-  RKX_CTRL_DEVICE = 0,     /* step control on the device; the one pow() of the new h
-                              (include/tools.h:226) is answered by a host service
-                              thread with the C library's pow, so every h equals the
-                              reference's bitwise (default) */
-  RKX_CTRL_HOST = 1,       /* step control on the host after each step (one host
-                              round trip per step; bitwise as well) */
-  RKX_CTRL_DEVICE_POW = 2  /* step control and pow entirely on the device: fastest,
-                              but the device pow may differ from the C library's by
-                              an ulp, which changes the trajectory at round-off level */
+  RKX_CTRL_DEVICE = 0,     /* step control on the device, in the last workgroup of each
+                              step; the one pow() of the new h (include/tools.h:226) is
+                              the C library's pow restated bitwise (rkx_pow_device), so
+                              every h equals the reference's (default) */
+  RKX_CTRL_HOST = 1        /* step control on the host after each step with the C
+                              library's pow (one host round trip per step; bitwise the
+                              same steps) */
 };
 
 typedef struct rkx_opts {
@@ -168,7 +166,7 @@ enum {
 
 typedef struct rkx_batch_opts {
   size_t hbm_budget;      /* device bytes the batch may hold for instance data (0: no limit) */
-  int ctrl;               /* RKX_CTRL_DEVICE or RKX_CTRL_DEVICE_POW */
+  int ctrl;               /* RKX_CTRL_DEVICE */
   int tile_waves;         /* fused-kernel strip width for all instances (0: chosen per size;
                              rk_amd/autotune.py picks both by timing the batch's own work) */
   int tile_rows;          /* fused-kernel band height (0: chosen per size; rounded up as
@@ -211,13 +209,25 @@ size_t rkx_batch_bytes(const rkx_batch *b);
  * `count` systems x' = c0 x - d0 x y, y' = d1 x y - c1 y, constants
  * params_dev[4k..4k+3] = (c0, d0, d1, c1), start y0_dev[2k..2k+1], each
  * integrated from t0 to te with its own step size, accept/reject and the
- * stage arithmetic of seq D; the step control uses the device pow (as
- * RKX_CTRL_DEVICE_POW).  h0 > 0 is the initial step of every system;
+ * stage arithmetic of seq D and the step control of RKX_CTRL_DEVICE, so each
+ * system's accepted/rejected sequence and result equal a plain single solve
+ * of that system bitwise.  h0 > 0 is the initial step of every system;
  * 1 <= count <= 2^30.  Writes
  * y_dev[2k..2k+1] and the per-system accepted/rejected counts (device). */
 int rkx_lv_ensemble(int count, const double *params_dev, const double *y0_dev, double *y_dev,
                     double t0, double te, const rkx_method *m, double tol, double h0,
                     int step_limit, int *acc_dev, int *rej_dev, void *stream);
+
+/* ---- the step control's pow ------------------------------------------------
+ * pow(x, y) of the C library (glibc 2.35: ARM optimized-routines' algorithm,
+ * its x86-64 FMA build), restated so the device step control reproduces the
+ * reference's h_new_acc/h_new_rej (include/tools.h:221-233) bitwise.  Domain:
+ * x >= 0 (subnormals, +0, +inf included), 0 < y <= 1/2.  Elementwise over n
+ * values: on the device (x_dev, y_dev, p_dev device pointers; synchronous)
+ * and on the host CPU (the same code compiled for the host). */
+int rkx_pow_device(const double *x_dev, const double *y_dev, double *p_dev, size_t n,
+                   void *stream);
+int rkx_pow_host(const double *x, const double *y, double *p, size_t n);
 
 /* ---- runtime -------------------------------------------------------------- */
 const char *rkx_last_error(void);

@@ -100,6 +100,7 @@ int orc_method_by_name(const char *name, orc_method *m);
 /* step control (exported for the synthetic-value tests) -------------------- */
 double orc_h_new(double h, double err, double tol, int ord);
 double orc_update_error_max(double emax, double ej, double yj, double yj_old);
+void orc_pow(const double *x, const double *y, double *p, size_t n);
 double orc_initial_stepsize(double H, double tol, int step_limit, double h0);
 
 /* solvers ------------------------------------------------------------------- */
@@ -115,6 +116,10 @@ int orc_solve_PipeDls(const orc_ode *p, const orc_method *m, double t0, double t
 int orc_solve_pthreads_D(const orc_ode *p, const orc_method *m, double t0, double te,
                          const double *y0, double *y, double tol, const orc_opts *o,
                          int threads, orc_stats *st);
+/* independent LV pairs with per-system constants, each through orc_solve_D */
+int orc_lv_ensemble(int count, const double *params, const double *y0, double *y, double t0,
+                    double te, const orc_method *m, double tol, double h0, int step_limit,
+                    int *acc, int *rej, int threads);
 /* accuracy reference: the D loop in long double at any tolerance (not a
  * parity reference); y0 and y are double, the state is long double */
 int orc_solve_ld(const orc_ode *p, const char *method, double t0, double te, const double *y0,

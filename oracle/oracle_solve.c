@@ -27,6 +27,14 @@ double orc_h_new(double h, double err, double tol, int ord)
   return h * fmin(2.0, fmax(0.5, 0.9 * pow(tol / err, 1.0 / (ord + 1.0))));
 }
 
+/* The C library's pow elementwise: the reference's one libm call
+ * (include/tools.h:226), as the checker of the device's restatement. */
+void orc_pow(const double *x, const double *y, double *p, size_t n)
+{
+  for (size_t i = 0; i < n; ++i)
+    p[i] = pow(x[i], y[i]);
+}
+
 double orc_update_error_max(double emax, double ej, double yj, double yj_old)
 {
   const double divisor = fmax(fabs(yj), fabs(yj_old));
@@ -532,4 +540,81 @@ int orc_solve_pthreads_D(const orc_ode *p, const orc_method *m, double t0, doubl
   free(tid);
   free(args);
   return 0;
+}
+
+/* ---- ensemble of independent LV pairs (checker of rkx_lv_ensemble) --------
+ * Each system k is the reference's LV problem (src/problems/lv.c:69-77) with
+ * one pair and its own constants params[4k..4k+3] = (c0, d0, d1, c1), solved
+ * by orc_solve_D from y0[2k..2k+1]; systems are shared out over `threads`
+ * POSIX threads in contiguous ranges. */
+typedef struct {
+  int first, count;
+  const double *params, *y0;
+  double *y;
+  double t0, te, tol, h0;
+  int step_limit;
+  const orc_method *m;
+  int *acc, *rej;
+  int rc;
+} ens_arg;
+
+static void *ens_worker(void *vp)
+{
+  ens_arg *a = (ens_arg *) vp;
+  orc_opts o;
+  memset(&o, 0, sizeof(o));
+  o.h0 = a->h0;
+  o.step_limit = a->step_limit;
+  for (int k = a->first; k < a->first + a->count && !a->rc; k++) {
+    orc_ode p;
+    const double *q = a->params + 4 * (size_t) k;
+    orc_ode_init_lv(&p, 1, q[0], q[1], q[2], q[3]);
+    orc_stats st;
+    a->rc = orc_solve_D(&p, a->m, a->t0, a->te, a->y0 + 2 * (size_t) k, a->y + 2 * (size_t) k,
+                        a->tol, &o, &st);
+    a->acc[k] = st.steps_acc;
+    a->rej[k] = st.steps_rej;
+  }
+  return NULL;
+}
+
+int orc_lv_ensemble(int count, const double *params, const double *y0, double *y, double t0,
+                    double te, const orc_method *m, double tol, double h0, int step_limit,
+                    int *acc, int *rej, int threads)
+{
+  if (count < 1 || threads < 1)
+    return -1;
+  if (threads > count)
+    threads = count;
+  ens_arg *args = (ens_arg *) calloc((size_t) threads, sizeof(ens_arg));
+  pthread_t *tid = (pthread_t *) malloc((size_t) threads * sizeof(pthread_t));
+  const int q = count / threads, r = count % threads;
+  for (int i = 0; i < threads; i++) {
+    ens_arg *a = &args[i];
+    a->first = i < r ? i * (q + 1) : r * (q + 1) + (i - r) * q;
+    a->count = i < r ? q + 1 : q;
+    a->params = params;
+    a->y0 = y0;
+    a->y = y;
+    a->t0 = t0;
+    a->te = te;
+    a->tol = tol;
+    a->h0 = h0;
+    a->step_limit = step_limit;
+    a->m = m;
+    a->acc = acc;
+    a->rej = rej;
+  }
+  for (int i = 1; i < threads; i++)
+    pthread_create(&tid[i], NULL, ens_worker, &args[i]);
+  ens_worker(&args[0]);
+  int rc = args[0].rc;
+  for (int i = 1; i < threads; i++) {
+    pthread_join(tid[i], NULL);
+    if (args[i].rc)
+      rc = args[i].rc;
+  }
+  free(args);
+  free(tid);
+  return rc;
 }

@@ -65,6 +65,8 @@ def lib():
         L.orc_method_by_name.argtypes = [C.c_char_p, C.POINTER(Method)]
         L.orc_h_new.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_h_new.restype = C.c_double
+        L.orc_pow.argtypes = [dp, dp, dp, C.c_size_t]
+        L.orc_pow.restype = None
         L.orc_update_error_max.argtypes = [C.c_double] * 4
         L.orc_update_error_max.restype = C.c_double
         L.orc_initial_stepsize.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double]
@@ -74,6 +76,10 @@ def lib():
         for fn in ("orc_solve_D", "orc_solve_PipeD", "orc_solve_PipeDls"):
             getattr(L, fn).argtypes = sig + [C.POINTER(Stats)]
         L.orc_solve_pthreads_D.argtypes = sig + [C.c_int, C.POINTER(Stats)]
+        ip = C.POINTER(C.c_int)
+        L.orc_lv_ensemble.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double,
+                                      C.POINTER(Method), C.c_double, C.c_double, C.c_int, ip, ip,
+                                      C.c_int]
         L.orc_solve_ld.argtypes = [C.POINTER(Ode), C.c_char_p, C.c_double, C.c_double, dp, dp,
                                    C.c_double, C.c_double, C.c_int, C.POINTER(Stats)]
         _lib = L
@@ -185,6 +191,35 @@ def solve_ld(p, meth, t0, te, y0, tol, h0=0.0, step_limit=0):
     if rc:
         raise RuntimeError("long-double oracle solve failed: %d" % rc)
     return y, dict(acc=st.steps_acc, rej=st.steps_rej, steps=st.steps_acc + st.steps_rej, t=st.t)
+
+
+def lv_ensemble(params, y0, t0, te, meth, tol, h0, step_limit=0, threads=None):
+    """Independent LV pairs, each solved by the oracle's seq D with its own
+    constants (rows of params: c0, d0, d1, c1).  Returns (y, acc, rej)."""
+    m = method(meth) if isinstance(meth, str) else meth
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    count = params.shape[0]
+    y = np.empty_like(y0)
+    acc = np.zeros(count, dtype=np.int32)
+    rej = np.zeros(count, dtype=np.int32)
+    threads = threads or min(16, os.cpu_count() or 1)
+    ip = C.POINTER(C.c_int)
+    rc = lib().orc_lv_ensemble(count, _dp(params), _dp(y0), _dp(y), t0, te, C.byref(m), tol, h0,
+                               step_limit, acc.ctypes.data_as(ip), rej.ctypes.data_as(ip),
+                               threads)
+    if rc:
+        raise RuntimeError("oracle ensemble failed: %d" % rc)
+    return y, acc, rej
+
+
+def pow_libm(x, y):
+    """The C library's pow elementwise (the reference's step-control pow)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    p = np.empty_like(x)
+    lib().orc_pow(_dp(x), _dp(y), _dp(p), x.size)
+    return p
 
 
 def footprint(y):

@@ -3,13 +3,12 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# RKX_LIB selects another build of the same C ABI (e.g. an experimental kernel)
-_LIB_FILE = os.environ.get("RKX_LIB") or os.path.join(HERE, "librkx.so")
+_LIB_FILE = os.path.join(HERE, "librkx.so")
 
 RKX_MAX_STAGES = 16
 RKX_BRUSS2D, RKX_LV, RKX_LV_CHAIN = 1, 2, 3
 RKX_IMPL_D, RKX_IMPL_PIPED, RKX_IMPL_PIPEDLS = 0, 1, 2
-RKX_CTRL_DEVICE, RKX_CTRL_HOST, RKX_CTRL_DEVICE_POW = 0, 1, 2
+RKX_CTRL_DEVICE, RKX_CTRL_HOST = 0, 1
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ELIMIT"}
 
@@ -86,6 +85,8 @@ SIGNATURES = [
     ("rkx_lv_ensemble", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                   C.c_double, C.POINTER(Method), C.c_double, C.c_double, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rkx_pow_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    ("rkx_pow_host", C.c_int, [_dp, _dp, _dp, C.c_size_t]),
     ("rkx_last_error", C.c_char_p, []),
     ("rkx_device_sync", C.c_int, []),
     ("rkx_solve_footprint", C.c_size_t, [C.POINTER(Ode), C.POINTER(Method), C.c_int]),

@@ -28,6 +28,7 @@ SOURCES = [
     "rkx_kernels_sweep_dopri87.hip",
     "rkx_kernels_lv.hip",
     "rkx_kernels_chain.hip",
+    "rkx_pow.hip",
 ]
 
 # Reference-style solver binaries over the C ABI (rk_amd/csrc/dropin): name ->

@@ -75,7 +75,6 @@ struct rkx_batch {
   size_t bytes = 0;
   bool dirty = true;
   int hi_slot = 0;                      // one past the highest slot ever used
-  pow_service_thread svc;
 };
 
 namespace {
@@ -221,36 +220,6 @@ int enqueue_batched_step(rkx_batch *b)
   return RKX_OK;
 }
 
-// copy every instance's state back; finish time-outs of the pow service on
-// the host; mark finished instances
-int poll_states(rkx_batch *b, int *newly_done)
-{
-  *newly_done = 0;
-  if (b->hi_slot == 0)
-    return RKX_OK;
-  RKX_HIP(hipMemcpyAsync(b->st_pinned, b->st_dev, sizeof(state) * b->hi_slot,
-                         hipMemcpyDeviceToHost, b->stream));
-  RKX_HIP(hipStreamSynchronize(b->stream));
-  for (int i = 0; i < b->hi_slot; ++i) {
-    batch_inst &I = b->inst[static_cast<size_t>(i)];
-    if (!I.used)
-      continue;
-    state s = b->st_pinned[i];
-    if (s.pow_wait) {
-      dev_coef hc{};
-      host_finish(s, true, I.dm, hc);
-      RKX_HIP(hipMemcpy(b->cf_dev + i, &hc, sizeof(hc), hipMemcpyHostToDevice));
-      RKX_HIP(hipMemcpy(b->st_dev + i, &s, sizeof(s), hipMemcpyHostToDevice));
-    }
-    I.snap = s;
-    if (s.done && !I.done) {
-      I.done = true;
-      ++*newly_done;
-    }
-  }
-  return RKX_OK;
-}
-
 void drop_graph(rkx_batch *b)
 {
   if (b->graph_exec)
@@ -311,8 +280,8 @@ int launch_chunk(rkx_batch *b, int slot, int k)
 }
 
 // Read the snapshot of `slot` once its chunk has finished: update the host
-// copies, count newly finished instances, report pending pow time-outs.
-int read_snapshot(rkx_batch *b, int slot, int *newly_done, bool *pow_wait)
+// copies and count newly finished instances.
+int read_snapshot(rkx_batch *b, int slot, int *newly_done)
 {
   RKX_HIP(hipEventSynchronize(b->ev[slot]));
   const state *snap = b->st_pinned + static_cast<size_t>(slot) * kBatchCap;
@@ -321,10 +290,6 @@ int read_snapshot(rkx_batch *b, int slot, int *newly_done, bool *pow_wait)
     if (!I.used)
       continue;
     const state &s = snap[i];
-    if (s.pow_wait) {
-      *pow_wait = true;
-      continue;
-    }
     I.snap = s;
     if (s.done && !I.done) {
       I.done = true;
@@ -355,8 +320,8 @@ int rkx_batch_create(rkx_batch **out, const rkx_batch_opts *o)
     opts = *o;
   else
     rkx_batch_opts_default(&opts);
-  if (opts.ctrl != RKX_CTRL_DEVICE && opts.ctrl != RKX_CTRL_DEVICE_POW)
-    return fail(RKX_EINVAL, "rkx_batch_create: ctrl must be RKX_CTRL_DEVICE or _DEVICE_POW");
+  if (opts.ctrl != RKX_CTRL_DEVICE)
+    return fail(RKX_EINVAL, "rkx_batch_create: ctrl must be RKX_CTRL_DEVICE");
   rkx_batch *b = new rkx_batch();
   b->opts = opts;
   b->stream = static_cast<hipStream_t>(opts.stream);
@@ -386,8 +351,6 @@ int rkx_batch_create(rkx_batch **out, const rkx_batch_opts *o)
     (void) hipGetLastError();
     rc = fail(RKX_ENOMEM, "rkx_batch_create: pinned allocation failed");
   }
-  if (!rc && opts.ctrl == RKX_CTRL_DEVICE)
-    rc = b->svc.start(kBatchCap);
   if (rc) {
     rkx_batch_destroy(b);
     return rc;
@@ -410,7 +373,6 @@ int rkx_batch_destroy(rkx_batch *b)
     return RKX_OK;
   if (b->stream)
     (void) hipStreamSynchronize(b->stream);
-  b->svc.stop();
   for (auto &I : b->inst)
     if (I.used)
       free_inst(I);
@@ -516,14 +478,9 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
   hs.ord = m->ord;
   hs.ctrl = b->opts.ctrl;
   hs.s = m->s;
-  hs.pow_spin = pow_spin_limit();
   hs.meth = b->dm_dev + slot;
   hs.coef = b->cf_dev + slot;
   hs.done = !(t0 < te);
-  if (b->opts.ctrl == RKX_CTRL_DEVICE) {
-    b->svc.reset_slot(slot);
-    hs.mb = b->svc.device_ptr(slot);
-  }
   dev_coef hc{};
   premult(I.dm, m->s, hs.h, hc);
   I.snap = hs;
@@ -590,30 +547,9 @@ int rkx_batch_run(rkx_batch *b, int max_steps, int *active)
       pending[nxt] = true;
     }
     int newly = 0;
-    bool pw = false;
-    if (int rc = read_snapshot(b, cur, &newly, &pw))
+    if (int rc = read_snapshot(b, cur, &newly))
       return rc;
     pending[cur] = false;
-    if (pw) {
-      // a pow request of some instance timed out: drain the stream (that
-      // instance's kernels return at once meanwhile), finish its step
-      // control on the host, and continue from the device's current states
-      RKX_HIP(hipStreamSynchronize(b->stream));
-      pending[nxt] = false;
-      int more = 0;
-      if (int rc = poll_states(b, &more))
-        return rc;
-      newly += more;
-      if (enqueued < max_steps && count_active() > 0) {
-        const int k = std::min(kPollSteps, max_steps - enqueued);
-        if (newly)
-          b->dirty = true;
-        if (int rc = launch_chunk(b, nxt, k))
-          return rc;
-        enqueued += k;
-        pending[nxt] = true;
-      }
-    }
     if (newly)
       b->dirty = true;
     cur = nxt;

@@ -7,8 +7,14 @@
 #pragma once
 
 #include "rkx_internal.hpp"
+#include "rkx_pow.hpp"
 
 namespace rkx {
+
+// Tables of libm_pow in constant memory (one copy per kernel translation unit).
+static __constant__ pow_tables dev_pow_tables =
+#include "rkx_pow_tables.inc"
+    ;
 
 // Vectors reached through an instance table are generic pointers to the
 // compiler, which then emits FLAT memory instructions; those count against
@@ -136,36 +142,10 @@ __device__ __forceinline__ void err_term(double e, double yn, double yo, double 
 
 // ---- step control (include/tools.h:217-278, 191-198) ------------------------
 
-// Posts pow(st->pow_x, st->pow_y) to the host service and waits, bounded, for
-// the answer.  Returns false on time-out.
-__device__ inline bool pow_service(state *st, double &p)
-{
-  pow_mailbox *mb = st->mb;
-  const unsigned long long seq = st->pow_seq + 1ull;
-  st->pow_seq = seq;
-  __hip_atomic_store(reinterpret_cast<unsigned long long *>(&mb->x),
-                     static_cast<unsigned long long>(__double_as_longlong(st->pow_x)),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(reinterpret_cast<unsigned long long *>(&mb->y),
-                     static_cast<unsigned long long>(__double_as_longlong(st->pow_y)),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&mb->req, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  const int spin = st->pow_spin;
-  for (int it = 0; it < spin; ++it) {
-    if (__hip_atomic_load(&mb->resp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) == seq) {
-      const unsigned long long pb = __hip_atomic_load(
-          reinterpret_cast<unsigned long long *>(&mb->p), __ATOMIC_RELAXED,
-          __HIP_MEMORY_SCOPE_SYSTEM);
-      p = __longlong_as_double(static_cast<long long>(pb));
-      return true;
-    }
-    __builtin_amdgcn_s_sleep(8);
-  }
-  return false;
-}
-
 // Runs in exactly one thread after every workgroup has folded its maximum into
 // st->err_bits / st->zero_div.  Resets the accumulators for the next step.
+// The new h uses libm_pow, the C library's pow restated bitwise, so the
+// device reproduces the reference's step sizes without a host round trip.
 __device__ inline void dev_control(state *st)
 {
   const unsigned long long bits = atomicOr(&st->err_bits, 0ull);
@@ -184,15 +164,7 @@ __device__ inline void dev_control(state *st)
     st->trace[3 * attempt + 2] = st->err;
   }
   const bool need = ctrl_decide(*st);
-  double p = 0.0;
-  if (need) {
-    if (st->ctrl == RKX_CTRL_DEVICE_POW) {
-      p = pow(st->pow_x, st->pow_y);
-    } else if (!pow_service(st, p)) {
-      st->pow_wait = 1;                // the host loop finishes this step's control
-      return;
-    }
-  }
+  const double p = need ? libm_pow(st->pow_x, st->pow_y, dev_pow_tables) : 0.0;
   ctrl_finish(*st, need, p);
   premult(*st->meth, st->s, st->h, *st->coef);
 }

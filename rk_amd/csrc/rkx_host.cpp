@@ -117,22 +117,6 @@ void fill_dev_method(const rkx_method &m, dev_method &d)
   }
 }
 
-// Iterations of the bounded wait for the host pow service (each one sleeps and
-// reads the mailbox over the host link, roughly a microsecond): about 0.2 s
-// in total, after which the step is handed to the host loop instead.
-constexpr int kPowSpin = 200000;
-
-int pow_spin_limit()
-{
-  const char *e = std::getenv("RKX_POW_SPIN");
-  if (e && *e) {
-    const long v = std::strtol(e, nullptr, 10);
-    if (v >= 0 && v <= kPowSpin)
-      return static_cast<int>(v);
-  }
-  return kPowSpin;
-}
-
 // initial_stepsize (src/tools.c:25-36)
 double initial_h(double H, double tol, int limit, double h0)
 {
@@ -179,78 +163,11 @@ struct graph_holder {
 
 }  // namespace
 
-int pow_service_thread::start(int nslots)
-{
-  void *p = nullptr;
-  const size_t bytes = sizeof(pow_mailbox) * static_cast<size_t>(nslots);
-  if (hipHostMalloc(&p, bytes, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
-    (void) hipGetLastError();
-    return fail(RKX_ENOMEM, "hipHostMalloc of the pow mailbox failed");
-  }
-  mb_ = static_cast<pow_mailbox *>(p);
-  std::memset(mb_, 0, bytes);
-  nslots_ = nslots;
-  handled_.assign(static_cast<size_t>(nslots), 0ull);
-  void *dp = nullptr;
-  RKX_HIP(hipHostGetDevicePointer(&dp, p, 0));
-  dev_ = static_cast<pow_mailbox *>(dp);
-  stop_.store(false);
-  th_ = std::thread([this] { run(); });
-  return RKX_OK;
-}
-
-void pow_service_thread::stop()
-{
-  stop_.store(true);
-  if (th_.joinable())
-    th_.join();
-  if (mb_)
-    (void) hipHostFree(mb_);
-  mb_ = nullptr;
-  dev_ = nullptr;
-}
-
-void pow_service_thread::reset_slot(int i)
-{
-  // A slot handed to a new solve starts from request number 0 again.  The
-  // service thread sees req change and answers once more (a harmless extra
-  // answer nobody waits for), then follows the new solve's numbers.
-  __atomic_store_n(&mb_[i].req, 0ull, __ATOMIC_RELEASE);
-  __atomic_store_n(&mb_[i].resp, 0ull, __ATOMIC_RELEASE);
-}
-
-void pow_service_thread::run()
-{
-  while (!stop_.load(std::memory_order_relaxed)) {
-    bool any = false;
-    for (int i = 0; i < nslots_; ++i) {
-      pow_mailbox *m = mb_ + i;
-      const unsigned long long r = __atomic_load_n(&m->req, __ATOMIC_ACQUIRE);
-      if (r != handled_[static_cast<size_t>(i)]) {
-        const double x = *reinterpret_cast<volatile double *>(&m->x);
-        const double y = *reinterpret_cast<volatile double *>(&m->y);
-        *reinterpret_cast<volatile double *>(&m->p) = std::pow(x, y);
-        __atomic_store_n(&m->resp, r, __ATOMIC_RELEASE);
-        handled_[static_cast<size_t>(i)] = r;
-        any = true;
-      }
-    }
-    if (!any)
-      __builtin_ia32_pause();
-  }
-}
-
-// Host half of the step control: ctrl_decide/ctrl_finish with the C library's
+// The step control on the host: ctrl_decide/ctrl_finish with the C library's
 // pow, then the premultiplied coefficients of the next step.
-void host_finish(state &st, bool decided, const dev_method &dm, dev_coef &hc)
+void host_finish(state &st, const dev_method &dm, dev_coef &hc)
 {
-  bool need;
-  if (!decided) {
-    need = ctrl_decide(st);
-  } else {
-    need = true;                       // decided on the device, waiting for pow
-    st.pow_wait = 0;
-  }
+  const bool need = ctrl_decide(st);
   const double p = need ? std::pow(st.pow_x, st.pow_y) : 0.0;
   ctrl_finish(st, need, p);
   premult(dm, st.s, st.h, hc);
@@ -487,7 +404,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     rkx_opts_default(&o);
   if (o.impl < RKX_IMPL_D || o.impl > RKX_IMPL_PIPEDLS)
     return fail(RKX_EINVAL, "rkx_solve: unknown impl");
-  if (o.ctrl < RKX_CTRL_DEVICE || o.ctrl > RKX_CTRL_DEVICE_POW)
+  if (o.ctrl != RKX_CTRL_DEVICE && o.ctrl != RKX_CTRL_HOST)
     return fail(RKX_EINVAL, "rkx_solve: unknown step-control mode");
   if (o.trace_cap < 0 || (o.trace_cap > 0 && !o.trace))
     return fail(RKX_EINVAL, "rkx_solve: trace_cap without a trace buffer");
@@ -550,7 +467,6 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   hs.ctrl = o.ctrl;
   hs.trace = trace_dev;
   hs.trace_cap = o.trace_cap;
-  hs.pow_spin = pow_spin_limit();
   hs.s = s;
   hs.meth = dm_dev;
   hs.coef = cf_dev;
@@ -558,13 +474,6 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   premult(dm, s, hs.h, hc);
   hs.cur = 0;
   hs.done = !(t0 < te);
-  pow_service_thread svc;
-  if (o.ctrl == RKX_CTRL_DEVICE && !hs.done) {
-    if (int rc = svc.start())
-      return rc;
-    hs.mb = svc.device_ptr();
-  }
-
   RKX_HIP(hipMemcpyAsync(dm_dev, &dm, sizeof(dm), hipMemcpyHostToDevice, stream));
   RKX_HIP(hipMemcpyAsync(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice, stream));
   RKX_HIP(hipMemcpyAsync(st_dev, &hs, sizeof(hs), hipMemcpyHostToDevice, stream));
@@ -649,7 +558,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
         o.trace[3 * attempt + 1] = cur.h;
         o.trace[3 * attempt + 2] = cur.err;
       }
-      host_finish(cur, false, dm, hc);
+      host_finish(cur, dm, hc);
       RKX_HIP(hipMemcpyAsync(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice, stream));
       hs_pinned[0] = cur;
       RKX_HIP(hipMemcpyAsync(st_dev, hs_pinned, sizeof(state), hipMemcpyHostToDevice, stream));
@@ -700,25 +609,6 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
       RKX_HIP(hipEventSynchronize(ev[a]));
       if (hs_pinned[a].done)
         break;
-      if (hs_pinned[a].pow_wait) {
-        // The pow service did not answer in time: drain the stream (the
-        // remaining kernels return at once), finish this step's control on
-        // the host and restart the pipeline.
-        RKX_HIP(hipStreamSynchronize(stream));
-        state cur;
-        RKX_HIP(hipMemcpy(&cur, st_dev, sizeof(state), hipMemcpyDeviceToHost));
-        host_finish(cur, true, dm, hc);
-        RKX_HIP(hipMemcpy(cf_dev, &hc, sizeof(hc), hipMemcpyHostToDevice));
-        RKX_HIP(hipMemcpy(st_dev, &cur, sizeof(state), hipMemcpyHostToDevice));
-        if (cur.done)
-          break;
-        if (int rc = launch_chunk(0))
-          return rc;
-        k = -1;
-        last_attempts = -1;
-        stalled = 0;
-        continue;
-      }
       // every chunk must advance the attempt counter; a chunk that does not
       // means the device-side step control is not running -- stop, do not spin
       const int att = hs_pinned[a].acc + hs_pinned[a].rej;

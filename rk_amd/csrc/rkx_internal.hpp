@@ -59,17 +59,6 @@ struct dev_coef {
   double col[RKX_MAX_STAGES][RKX_MAX_STAGES + 2];
 };
 
-// Mailbox of the host pow service, in fine-grained pinned host memory.  The
-// step control posts x, y and a new sequence number in req; the host service
-// thread answers p = pow(x, y) with the C library and echoes the sequence
-// number in resp.
-struct pow_mailbox {
-  unsigned long long req;
-  unsigned long long resp;
-  double x, y, p;
-  unsigned long long pad[3];
-};
-
 // Integration state of one solve, resident in device memory.  Written only by
 // the step control (the last workgroup of a step, or the host).
 struct alignas(64) state {
@@ -84,13 +73,9 @@ struct alignas(64) state {
   int step_limit;
   int ord;
   int ctrl;                          // RKX_CTRL_*
-  int pow_wait;                      // 1: the new h waits for pow(pow_x, pow_y) from the host
   int trace_cap;
   int s;                             // stages
-  int pow_spin;                      // bound of the wait for the host pow service (iterations)
-  double pow_x, pow_y;               // pending pow arguments
-  unsigned long long pow_seq;        // last request number posted to the mailbox
-  pow_mailbox *mb;                   // host pow service mailbox (device-visible pointer)
+  double pow_x, pow_y;               // arguments of the step's pow (ctrl_decide)
   double *trace;                     // device buffer, 3 doubles per attempt
   const dev_method *meth;            // raw coefficients for the premultiplication
   dev_coef *coef;                    // premultiplied coefficients of the next step
@@ -154,37 +139,10 @@ int check_ode(const rkx_ode *p);
 int check_method(const rkx_method *m);
 void fill_dev_method(const rkx_method &m, dev_method &d);
 double initial_h(double H, double tol, int limit, double h0);   // src/tools.c:25-36
-// Iterations the device step control waits for the host pow service before
-// handing the step to the host (RKX_POW_SPIN overrides; 0 makes every step
-// take the host path, which tests that path).
-int pow_spin_limit();
 int default_stream(hipStream_t *out);
-// the host half of the step control (ctrl_decide/ctrl_finish with the C
-// library's pow) and the next step's premultiplied coefficients
-void host_finish(state &st, bool decided, const dev_method &dm, dev_coef &hc);
-
-// Host pow service: answers pow(x, y) requests that device step controls post
-// in mailboxes of fine-grained pinned host memory (one slot per concurrent
-// solve), with the C library's pow -- the one libm call of the reference's
-// step control (tools.h:226) -- so device-side control reproduces the
-// reference's step sizes bitwise.  One busy-polling host thread.
-class pow_service_thread {
- public:
-  int start(int nslots = 1);
-  void stop();
-  void reset_slot(int i);
-  pow_mailbox *device_ptr(int slot = 0) const { return dev_ ? dev_ + slot : nullptr; }
-  ~pow_service_thread() { stop(); }
-
- private:
-  void run();
-  pow_mailbox *mb_ = nullptr;
-  pow_mailbox *dev_ = nullptr;
-  int nslots_ = 0;
-  std::vector<unsigned long long> handled_;
-  std::atomic<bool> stop_{false};
-  std::thread th_;
-};
+// the step control on the host (ctrl_decide/ctrl_finish with the C library's
+// pow) and the next step's premultiplied coefficients
+void host_finish(state &st, const dev_method &dm, dev_coef &hc);
 
 // Kernel launchers (rkx_kernels_*.hip).  All return hipError_t of the launch.
 hipError_t launch_eval_rng(const dev_ode &p, int i, int j, const double *y, double *f,

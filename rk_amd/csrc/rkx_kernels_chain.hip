@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
   const int2 td = tiles ? tiles[tile] : make_int2(0, tile);
   const chain_inst &I = insts[td.x];
   state *st = I.st;
-  if (st->done | st->pow_wait)
+  if (st->done)
     return;
   const int cur = st->cur;
   const double2 *__restrict__ y = reinterpret_cast<const double2 *>(cur ? I.y1 : I.y0);

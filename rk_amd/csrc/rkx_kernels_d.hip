@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_d_first(dev_ode p, const dev_coef *_
                                                     double *__restrict__ dy,
                                                     double *__restrict__ err)
 {
-  if (st->done | st->pow_wait)
+  if (st->done)
     return;
   const int cell = blockIdx.x * kBlock + threadIdx.x;
   if (cell >= p.cells)
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_d_interm(dev_ode p, const dev_coef *
                                                      double *__restrict__ dy,
                                                      double *__restrict__ err)
 {
-  if (st->done | st->pow_wait)
+  if (st->done)
     return;
   const int cell = blockIdx.x * kBlock + threadIdx.x;
   if (cell >= p.cells)
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_d_last(dev_ode p, const dev_coef *__
 {
   __shared__ double red[kBlock / 64];
   __shared__ unsigned int zred[kBlock / 64];
-  if (st->done | st->pow_wait)
+  if (st->done)
     return;                       // uniform: every block of the grid returns
   const int cur = st->cur;
   const double *y = cur ? y1 : y0;

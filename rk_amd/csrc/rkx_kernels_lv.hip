@@ -4,8 +4,10 @@
 // accept/reject.  The arithmetic of every attempted step is that of the
 // reference's seq D (block-inline.h:253-320, tools.h:191-288) on a system of
 // size 2, so each system's result equals a plain single solve of that system
-// with the same step-control mode.  The step control runs in the thread with
-// the device pow (thousands of systems step at once; no host round trip).
+// (identical accepted/rejected sequences and bitwise the same y).  The step
+// control runs in the thread, with libm_pow: the C library's pow restated, so
+// every new h equals the reference's (thousands of systems step at once; no
+// host round trip).
 #include "rkx_device.hpp"
 
 namespace rkx {
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(kLvBlock) void k_lv_ensemble(int count, lv_method m
       y[0] = yn[0];
       y[1] = yn[1];
     }
-    ctrl_finish(st, need, need ? pow(st.pow_x, st.pow_y) : 0.0);
+    ctrl_finish(st, need, need ? libm_pow(st.pow_x, st.pow_y, dev_pow_tables) : 0.0);
     ++attempts;
   }
   yout[2 * k] = y[0];

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   const int2 td = tiles ? tiles[tile] : make_int2(0, tile);
   const sweep_inst &I = insts[td.x];
   state *st = I.st;
-  if (st->done | st->pow_wait)
+  if (st->done)
     return;                                      // uniform over the instance's tiles
   const int cur = st->cur;
   gconst *__restrict__ y = as_global(cur ? I.y1 : I.y0);

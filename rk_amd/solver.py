@@ -9,12 +9,12 @@ import os
 
 import numpy as np
 
-from ._lib import (RKX_CTRL_DEVICE, RKX_CTRL_DEVICE_POW, RKX_CTRL_HOST, RKX_IMPL_D, RKX_IMPL_PIPED, RKX_IMPL_PIPEDLS,
+from ._lib import (RKX_CTRL_DEVICE, RKX_CTRL_HOST, RKX_IMPL_D, RKX_IMPL_PIPED, RKX_IMPL_PIPEDLS,
                    Opts, Stats, check, lib)
 from .methods import method as _method
 
 IMPLS = {"D": RKX_IMPL_D, "PipeD": RKX_IMPL_PIPED, "PipeDls": RKX_IMPL_PIPEDLS}
-CTRLS = {"device": RKX_CTRL_DEVICE, "host": RKX_CTRL_HOST, "device_pow": RKX_CTRL_DEVICE_POW}
+CTRLS = {"device": RKX_CTRL_DEVICE, "host": RKX_CTRL_HOST}
 
 
 def _ptr(x):
