@@ -75,10 +75,14 @@ int rkx_ode_start_seeded(const rkx_ode *p, unsigned long long seed, int instance
 int rkx_eval_rng_device(const rkx_ode *p, int i, int j, double t, const double *y_dev,
                         double *f_dev, void *stream);
 
-/* The same on host buffers (y: all n components; writes f[i..j] only),
- * staged through device memory: the host-side ode_eval_rng / ode_eval_comp
- * callbacks (ode.h:34-35) of the drop-in problem plugins.  Synchronous. */
+/* The same on host buffers (y: all n components; writes f[i..j] only): the
+ * host-side ode_eval_rng callback (ode.h:35) of the drop-in problem plugins.
+ * Only the components the stencils read (i - d - 1 .. j + d + 1, d the
+ * access distance) are staged through device memory.  Synchronous. */
 int rkx_eval_rng_host(const rkx_ode *p, int i, int j, double t, const double *y, double *f);
+
+/* ode_eval_comp(i, t, y) (ode.h:34) on host buffers: f_i into *out. */
+int rkx_eval_comp_host(const rkx_ode *p, int i, double t, const double *y, double *out);
 
 /* ---- methods (replaces rk_method_t, include/methods.h:92-118) ------------ */
 typedef struct rkx_method {
@@ -94,8 +98,11 @@ int rkx_method_get(const char *name, rkx_method *m);
 /* ---- single solve (replaces solver(), include/solver.h:47) --------------- */
 enum {
   RKX_IMPL_D = 0,        /* unfused stage kernels, full storage (src/impl/seq/D.c) */
-  RKX_IMPL_PIPED = 1,    /* stage-fused sweep, full storage of the step's output pair */
-  RKX_IMPL_PIPEDLS = 2   /* stage-fused sweep, low storage: two vectors of n */
+  RKX_IMPL_PIPED = 1,    /* the same stage-fused sweep as RKX_IMPL_PIPEDLS: the fused form
+                            keeps every stage vector of src/impl/seq/PipeD.c in registers,
+                            so it has no full-storage variant (two vectors of n) */
+  RKX_IMPL_PIPEDLS = 2   /* stage-fused sweep, low storage: two vectors of n
+                            (src/impl/seq/PipeDls.c) */
 };
 
 enum {

@@ -8,6 +8,14 @@ flags (Makefile:46: -O3 -g -D_GNU_SOURCE -D_REENTRANT, -lpthread -lm) and the
 compile-time configuration as -D defines (include/config.h).  Nothing is
 copied out of the reference tree; outputs go to oracle/_ref/ only.
 
+The drop-in proof (REF_DROPIN) links the reference's own driver
+(src/main.c) and problem object (src/problems/<problem>.c), compiled the same
+way, with this build's solver(), method functions and device problem
+descriptor (rk_amd/csrc/dropin: solver_hip.c, methods_dropin.c,
+desc_<problem>.c, all with the same -D defines) over rk_amd/librkx.so -- the
+rule INTEGRATION.md gives a maintainer of the reference.  Those binaries are
+built only when librkx.so exists.
+
 Usage: python oracle/build_ref.py [--ref /root/reference]
 Does nothing (exit 0) when the reference tree is absent.
 """
@@ -15,6 +23,7 @@ import argparse
 import os
 import subprocess
 import sys
+import tempfile
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -29,6 +38,8 @@ CONFIGS = {
         dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
     "seq_bruss_D_n128_verner65_tol1e-6": ("seq", "bruss2d-mix", "D",
         dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="VERNER65")),
+    "seq_bruss_D_n128_verner65_tol1e-8": ("seq", "bruss2d-mix", "D",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-8", H_INIT="1E-3", STEP_LIMIT=0, METHOD="VERNER65")),
     "seq_bruss_D_n128_defaults": ("seq", "bruss2d-mix", "D", dict(BRUSS_GRID_SIZE=128)),
     "seq_lv_D_10000_dopri54": ("seq", "lv", "D",
         dict(LV_COUNT=10000, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
@@ -64,6 +75,60 @@ for prob, d, meth, te, tol, h0 in DUMP_CASES:
     CONFIGS[name] = ("seq", prob, "D", defs)
 
 
+# name -> (reference problem source, drop-in descriptor, defines); each
+# mirrors a known answer of tests/golden/kat.json
+REF_DROPIN = {
+    "refmain_bruss_c1": ("bruss2d-mix", "bruss2d",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "refmain_bruss_defaults": ("bruss2d-mix", "bruss2d", dict(BRUSS_GRID_SIZE=128)),
+    "refmain_bruss_verner65_tol1e-8": ("bruss2d-mix", "bruss2d",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-8", H_INIT="1E-3", STEP_LIMIT=0, METHOD="VERNER65")),
+    "refmain_lv_dopri54": ("lv", "lv",
+        dict(LV_COUNT=10000, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "refmain_lv_verner65_t20": ("lv", "lv",
+        dict(LV_COUNT=10000, T_END="20.0", TOL="1E-8", STEP_LIMIT=0, METHOD="VERNER65")),
+}
+REPO = os.path.dirname(HERE)
+DROPIN_DIR = os.path.join(REPO, "rk_amd", "csrc", "dropin")
+LIBRKX = os.path.join(REPO, "rk_amd", "librkx.so")
+
+
+def build_ref_dropin(ref, name, prob, desc, defs):
+    """The reference's main.o and problem object + this build's solver,
+    methods and descriptor objects + librkx.so (found through an rpath
+    relative to the binary, so the tree can move)."""
+    with tempfile.TemporaryDirectory() as objdir:
+        return _link_ref_dropin(ref, name, prob, desc, defs, objdir)
+
+
+def _link_ref_dropin(ref, name, prob, desc, defs, objdir):
+    out = os.path.join(OUT, name)
+    dflags = ["-D%s=%s" % (k, v) for k, v in defs.items()]
+    base = ["gcc", "-I" + os.path.join(ref, "include"), "-D_REENTRANT", "-O3", "-g",
+            "-D_GNU_SOURCE"] + dflags
+    objs = []
+    for src in (os.path.join(ref, "src", "main.c"),
+                os.path.join(ref, "src", "problems", prob + ".c")):
+        o = os.path.join(objdir, os.path.basename(src)[:-2] + ".o")
+        r = subprocess.run(base + ["-c", src, "-o", o], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("compile of %s for %s failed:\n%s" % (src, name, r.stderr[-2000:]))
+        objs.append(o)
+    r = subprocess.run(["make", "-s", "-C", DROPIN_DIR, "objs", "PROBLEM=" + desc,
+                        "DEFINES=" + " ".join("%s=%s" % kv for kv in defs.items()),
+                        "OBJDIR=" + objdir], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in objects for %s failed:\n%s" % (name, r.stdout + r.stderr))
+    objs += [os.path.join(objdir, f) for f in ("solver_hip.o", "methods_dropin.o",
+                                               "desc_%s.o" % desc)]
+    cmd = ["gcc", "-o", out] + objs + ["-L" + os.path.dirname(LIBRKX), "-lrkx",
+                                       "-Wl,-rpath,$ORIGIN/../../rk_amd", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link of %s failed:\n%s" % (name, r.stderr[-2000:]))
+    return name
+
+
 def build_one(ref, name, kind, prob, impl, defs):
     out = os.path.join(OUT, name)
     srcs = [os.path.join(ref, "src", f) for f in ("main.c", "tools.c", "methods.c", "threads.c")]
@@ -88,11 +153,19 @@ def main():
         return 0
     os.makedirs(OUT, exist_ok=True)
     names = list(CONFIGS) if not a.only else a.only.split(",")
+    names = [n for n in names if n in CONFIGS]
+    dropins = [n for n in (list(REF_DROPIN) if not a.only else a.only.split(","))
+               if n in REF_DROPIN]
+    if dropins and not os.path.exists(LIBRKX):
+        print("rk_amd/librkx.so not built: skipping the reference-driver drop-in binaries")
+        dropins = []
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         futs = [ex.submit(build_one, a.ref, n, *CONFIGS[n]) for n in names]
+        futs += [ex.submit(build_ref_dropin, a.ref, n, *REF_DROPIN[n]) for n in dropins]
         for f in futs:
             f.result()
-    print("built %d reference binaries into %s" % (len(names), OUT))
+    print("built %d reference binaries (%d of them reference-driver drop-ins) into %s"
+          % (len(names) + len(dropins), len(dropins), OUT))
     return 0
 
 

@@ -64,6 +64,7 @@ SIGNATURES = [
     ("rkx_eval_rng_device", C.c_int, [C.POINTER(Ode), C.c_int, C.c_int, C.c_double, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
     ("rkx_eval_rng_host", C.c_int, [C.POINTER(Ode), C.c_int, C.c_int, C.c_double, _dp, _dp]),
+    ("rkx_eval_comp_host", C.c_int, [C.POINTER(Ode), C.c_int, C.c_double, _dp, _dp]),
     ("rkx_method_get", C.c_int, [C.c_char_p, C.POINTER(Method)]),
     ("rkx_opts_default", None, [C.POINTER(Opts)]),
     ("rkx_solve", C.c_int, [C.POINTER(Ode), C.POINTER(Method), C.c_double, C.c_double,

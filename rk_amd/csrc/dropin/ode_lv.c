@@ -1,7 +1,8 @@
 /*
  * Lotka-Volterra problem object (src/problems/lv.c) for the drop-in build:
  * LV_COUNT identical pairs x' = 0.1x - 0.2xy, y' = 0.4xy - 0.2y starting at
- * (0.2, 1.1) (lv.c:31-37).  Banner of lv.c:51-53.
+ * (0.2, 1.1) (lv.c:31-37).  Banner of lv.c:51-53.  The device descriptor
+ * comes from desc_lv.c; the ode_eval_* callbacks evaluate on the device.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -9,14 +10,6 @@
 #include "rkx_dropin.h"
 
 int ode_size = 2 * LV_COUNT;
-
-void rkx_dropin_problem(rkx_ode *p)
-{
-  if (rkx_ode_lv(p, LV_COUNT, 0.1, 0.2, 0.4, 0.2) != RKX_OK) {
-    fprintf(stderr, "rkx: %s\n", rkx_last_error());
-    exit(1);
-  }
-}
 
 void ode_start(double t, double *y0)
 {
@@ -42,12 +35,13 @@ void ode_eval_rng(int i, int j, double t, const double *y, double *f)
 
 double ode_eval_comp(int i, double t, const double *y)
 {
-  double *f = malloc(sizeof(double) * (size_t) ode_size);
-  if (!f)
-    abort();
-  ode_eval_rng(i, i, t, y, f);
-  const double v = f[i];
-  free(f);
+  rkx_ode p;
+  rkx_dropin_problem(&p);
+  double v;
+  if (rkx_eval_comp_host(&p, i, t, y, &v) != RKX_OK) {
+    fprintf(stderr, "rkx: %s\n", rkx_last_error());
+    exit(1);
+  }
   return v;
 }
 

@@ -18,7 +18,8 @@
 static const char *variant_name(int impl)
 {
   return impl == RKX_IMPL_D ? "hipD (stage kernels of D on the device)"
-       : impl == RKX_IMPL_PIPED ? "hipPipeD (stage-fused temporally blocked sweep)"
+       : impl == RKX_IMPL_PIPED ? "hipPipeD (stage-fused temporally blocked sweep, "
+                                  "two vectors resident; the PipeDls kernel)"
                                 : "hipPipeDls (stage-fused temporally blocked sweep, "
                                   "two vectors resident)";
 }

@@ -355,22 +355,62 @@ int rkx_eval_rng_device(const rkx_ode *p, int i, int j, double t, const double *
 
 int rkx_eval_rng_host(const rkx_ode *p, int i, int j, double t, const double *y, double *f)
 {
+  (void) t;
   if (int rc = check_ode(p))
     return rc;
   if (i < 0 || j >= p->n || j < i || !y || !f)
     return fail(RKX_EINVAL, "rkx_eval_rng_host: bad range or pointer");
-  const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
-  const size_t cnt = static_cast<size_t>(j - i + 1) * sizeof(double);
-  solve_mem mem;
-  double *yd = nullptr, *fd = nullptr;
-  if (int rc = mem.alloc(reinterpret_cast<void **>(&yd), vec))
+  // Only the components the stencils of f[i..j] read travel to the device:
+  // [i - d - 1, j + d + 1] clipped to the vector (the periodic chain reads
+  // across its ends, so it sends everything).  The kernel indexes absolutely,
+  // so it gets the window's base shifted back by lo, as the reference's tiled
+  // stages pass v - first to ode_eval_rng (include/block-inline.h:338).
+  const int d = rkx_ode_acc_dist(p);
+  int lo = 0, hi = p->n - 1;
+  if (p->kind != RKX_LV_CHAIN) {
+    lo = std::max(0, (i & ~1) - d);
+    hi = std::min(p->n - 1, (j | 1) + d);
+  }
+  const size_t win = static_cast<size_t>(hi - lo + 1);
+  // per-thread device staging, grown on demand and kept for the thread's
+  // later calls (a problem object's callbacks are called many times)
+  thread_local double *yd = nullptr, *fd = nullptr;
+  thread_local size_t cap = 0;
+  if (win > cap) {
+    if (yd)
+      (void) hipFree(yd);
+    if (fd)
+      (void) hipFree(fd);
+    yd = fd = nullptr;
+    cap = 0;
+    if (hipMalloc(reinterpret_cast<void **>(&yd), win * sizeof(double)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&fd), win * sizeof(double)) != hipSuccess) {
+      (void) hipGetLastError();
+      return fail(RKX_ENOMEM, "rkx_eval_rng_host: device staging allocation failed");
+    }
+    cap = win;
+  }
+  hipStream_t s = nullptr;
+  if (int rc = default_stream(&s))
     return rc;
-  if (int rc = mem.alloc(reinterpret_cast<void **>(&fd), vec))
+  RKX_HIP(hipMemcpyAsync(yd, y + lo, win * sizeof(double), hipMemcpyHostToDevice, s));
+  const dev_ode dp = make_dev_ode(*p);
+  RKX_HIP(launch_eval_rng(dp, i, j, yd - lo, fd - lo, s));
+  RKX_HIP(hipMemcpyAsync(f + i, fd + (i - lo), static_cast<size_t>(j - i + 1) * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  RKX_HIP(hipStreamSynchronize(s));
+  return RKX_OK;
+}
+
+int rkx_eval_comp_host(const rkx_ode *p, int i, double t, const double *y, double *out)
+{
+  if (!out)
+    return fail(RKX_EINVAL, "rkx_eval_comp_host: null output");
+  double v[2] = {0.0, 0.0};
+  // a one-component range written at absolute index i into v - (i & ~1)
+  if (int rc = rkx_eval_rng_host(p, i, i, t, y, v - (i & ~1)))
     return rc;
-  RKX_HIP(hipMemcpy(yd, y, vec, hipMemcpyHostToDevice));
-  if (int rc = rkx_eval_rng_device(p, i, j, t, yd, fd, nullptr))
-    return rc;
-  RKX_HIP(hipMemcpy(f + i, fd + i, cnt, hipMemcpyDeviceToHost));
+  *out = v[i & 1];
   return RKX_OK;
 }
 

@@ -91,6 +91,27 @@ def test_dropin_binaries_built():
         assert "rk_amd/bin/../librkx.so" in ldd
 
 
+def test_reference_driver_binaries_link_the_gpu_solver():
+    """oracle/_ref/refmain_*: main() and the ode_* plugin come from the
+    reference's own objects, solver(), the method functions and
+    rkx_dropin_problem() from rk_amd/csrc/dropin, over rk_amd/librkx.so."""
+    import subprocess
+    ref = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                       "_ref")
+    names = [f for f in os.listdir(ref) if f.startswith("refmain_")] if os.path.isdir(ref) else []
+    if not names:
+        pytest.skip("reference-driver binaries not built (no reference tree at build time)")
+    for name in names:
+        path = os.path.join(ref, name)
+        syms = subprocess.run(["nm", path], capture_output=True, text=True).stdout
+        for s in ("main", "solver", "rkx_dropin_problem", "ode_start", "ode_eval_comp",
+                  "ode_size", "DOPRI54", "VERNER65"):
+            assert re.search(r" [TDB] %s$" % s, syms, re.M), (name, s)
+        assert "rkx_solve_host" in syms
+        ldd = subprocess.run(["ldd", path], capture_output=True, text=True).stdout
+        assert "oracle/_ref/../../rk_amd/librkx.so" in ldd, ldd
+
+
 def test_serving_summary_statistics():
     """Latency percentiles and throughput of the serving summary (host logic)."""
     from rk_amd.serve import Request, summarize

@@ -83,6 +83,32 @@ def test_bruss_rhs_bitwise(rk, orc, N):
     assert np.array_equal(got[i:j + 1], ref[i:j + 1]) and not got[:i].any()
 
 
+@pytest.mark.parametrize("kind,grid", [("bruss2d", 2), ("bruss2d", 17), ("lv", 9),
+                                       ("lvchain", 11)])
+def test_host_callbacks_bitwise(rk, orc, kind, grid):
+    """ode_eval_comp / ode_eval_rng on host buffers (rkx_eval_comp_host,
+    rkx_eval_rng_host: only the stencil window is staged) equal the oracle's
+    bitwise for every component and for ranges at both ends and inside."""
+    import ctypes as C
+    from rk_amd._lib import check, lib
+    rng = np.random.default_rng(grid)
+    p = gpu_problem(rk, kind, grid) if kind != "lvchain" else rk.LVChain(grid, diff=0.02)
+    po = orc.make_ode(kind, grid, diff=0.02)
+    y = rng.uniform(0.1, 3.0, p.n)
+    dp = C.POINTER(C.c_double)
+    for i in range(p.n):
+        v = C.c_double()
+        check(lib().rkx_eval_comp_host(C.byref(p.desc), i, 0.0, y.ctypes.data_as(dp), C.byref(v)))
+        assert v.value == orc.eval_comp(po, y, i), i
+    for i, j in ((0, 0), (0, p.n - 1), (1, 4), (p.n - 3, p.n - 1), (p.n // 2 - 1, p.n // 2 + 2)):
+        f = np.full(p.n, -7.0)
+        check(lib().rkx_eval_rng_host(C.byref(p.desc), i, j, 0.0, y.ctypes.data_as(dp),
+                                      f.ctypes.data_as(dp)))
+        ref = orc.eval_rng(po, y, i, j)
+        assert np.array_equal(f[i:j + 1], ref[i:j + 1]), (i, j)
+        assert (f[:i] == -7.0).all() and (f[j + 1:] == -7.0).all()
+
+
 @pytest.mark.parametrize("kind", ["lv", "lvchain"])
 def test_lv_rhs_bitwise(rk, orc, kind):
     import torch
