@@ -13,9 +13,12 @@ step).  Algorithmic bytes per launch = 16 n (read y once, write y_new once);
 achieved = 16 n / (timed GPU time / attempted steps), timed with HIP events
 on the stream the kernels run on.
 cpu_baseline: the reference's pthreads PipeD4 (built from the reference's C
-sources into oracle/_ref) on the host's cores for a 200-step sample of the
+sources into oracle/_ref) on 16 host threads for a 200-step sample of the
 same workload, or the oracle's pthreads D restatement when that binary is
-absent.
+absent.  Beside it (cpu_reference): the reference's pthreads D (200 steps)
+and seq D (one thread, 20 steps) on the same workload, config 1 (N=128,
+t in [0, 4]) in full with seq D, pthreads D and PipeD4, and the GPU's time
+for config 1 (c1_gpu); host: CPU model, logical CPUs and physical cores.
 batched / ensemble / serving: the batched solves/s against plain per-instance
 launches on the 512-instance mixed BRUSS2D batch, the LV ensembles, and the
 continuous-batching server's p50/p99 latency and solves/s against the plain
@@ -49,19 +52,65 @@ def traffic_from_profile():
         return None
 
 
-def cpu_baseline(threads):
-    ref = os.path.join(REPO, "oracle", "_ref", "pthreads_bruss_PipeD4_n2000_dopri54_sl200")
-    if os.path.exists(ref):
-        env = dict(os.environ, NUM_THREADS=str(threads))
-        out = subprocess.run([ref], capture_output=True, text=True, env=env, timeout=600,
-                             cwd="/tmp").stdout
-        m = re.search(r"Kernel time: (\S+) s \((\S+) s per step\)", out)
-        steps = re.search(r"Number of steps: (\d+)", out)
-        per_step = float(m.group(2))
-        return {"value": 1.0 / per_step, "unit": "steps/s", "cores": threads,
+def _run_ref(name, threads=1):
+    """Run one reference binary of oracle/_ref (built from the reference's C
+    sources) and parse its own timer and step lines (src/tools.c:62-79)."""
+    ref = os.path.join(REPO, "oracle", "_ref", name)
+    if not os.path.exists(ref):
+        return None
+    env = dict(os.environ, NUM_THREADS=str(threads))
+    out = subprocess.run([ref], capture_output=True, text=True, env=env, timeout=900,
+                         cwd="/tmp").stdout
+    m = re.search(r"Kernel time: (\S+) s \((\S+) s per step\)", out)
+    steps = re.search(r"Number of steps: (\d+)", out)
+    return {"kernel_s": float(m.group(1)), "s_per_step": float(m.group(2)),
+            "steps": int(steps.group(1)), "threads": threads}
+
+
+def host_info():
+    """CPU model, logical CPUs and physical cores (lscpu) of the host the CPU
+    baselines run on."""
+    info = {"cpu_model": "", "logical_cpus": os.cpu_count(), "physical_cores": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        f = dict(re.findall(r"^([^:\n]+):\s*(.+)$", out, re.M))
+        info["cpu_model"] = f.get("Model name", "").strip()
+        info["physical_cores"] = int(f["Core(s) per socket"]) * int(f["Socket(s)"])
+        info["threads_per_core"] = int(f.get("Thread(s) per core", "1"))
+    except (OSError, subprocess.SubprocessError, KeyError, ValueError):
+        pass
+    try:
+        info["cpus_allowed"] = len(os.sched_getaffinity(0))
+    except OSError:
+        pass
+    return info
+
+
+def cpu_leg(threads):
+    """The reference's CPU solvers on this host, built from its C sources:
+    the 2000x2000 workload (pthreads PipeD4 and pthreads D on `threads`
+    threads, first 200 attempted steps; seq D on one thread, first 20) and
+    config 1 (N=128, t in [0, 4]) in full, seq D / pthreads D / PipeD4."""
+    c2 = {"pthreads_PipeD4": _run_ref("pthreads_bruss_PipeD4_n2000_dopri54_sl200", threads),
+          "pthreads_D": _run_ref("pthreads_bruss_D_n2000_dopri54_sl200", threads),
+          "seq_D": _run_ref("seq_bruss_D_n2000_dopri54_sl20", 1)}
+    c1 = {"seq_D": _run_ref("seq_bruss_D_n128_dopri54_tol1e-6", 1),
+          "pthreads_D": _run_ref("pthreads_bruss_D_n128_dopri54_tol1e-6", threads),
+          "pthreads_PipeD4": _run_ref("pthreads_bruss_PipeD4_n128_dopri54_tol1e-6", threads)}
+    return {"c2_sample": c2, "c1_full": c1}
+
+
+def cpu_baseline(leg, threads):
+    """The headline CPU baseline: pthreads PipeD4 (the reference's fastest
+    CPU form) on the 2000x2000 workload, or the oracle's pthreads D
+    restatement when the reference binaries are absent."""
+    r = leg["c2_sample"]["pthreads_PipeD4"]
+    if r is not None:
+        return {"value": 1.0 / r["s_per_step"], "unit": "steps/s", "cores": threads,
                 "kind": "reference",
-                "sample": "pthreads PipeD4 (reference C sources), BRUSS2D N=2000 DOPRI54 tol 1e-6 "
-                          "h0 1e-3, first %s attempted steps (STEP_LIMIT=200)" % steps.group(1)}
+                "sample": "pthreads PipeD4 (reference C sources), BRUSS2D N=2000 DOPRI54 tol "
+                          "1e-6 h0 1e-3, first %d attempted steps (STEP_LIMIT=200), %d threads"
+                          % (r["steps"], threads)}
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
     p = pyoracle.make_ode("bruss2d", N_GRID)
@@ -74,30 +123,22 @@ def cpu_baseline(threads):
                       "attempted steps"}
 
 
-def cpu_single_thread():
-    """The reference's seq D (one thread) on the first 20 attempted steps of
-    the same workload, when that binary was built (oracle/_ref)."""
-    ref = os.path.join(REPO, "oracle", "_ref", "seq_bruss_D_n2000_dopri54_sl20")
-    if not os.path.exists(ref):
-        return None
-    out = subprocess.run([ref], capture_output=True, text=True, timeout=600, cwd="/tmp").stdout
-    m = re.search(r"Kernel time: (\S+) s \((\S+) s per step\)", out)
-    steps = re.search(r"Number of steps: (\d+)", out)
-    return {"value": 1.0 / float(m.group(2)), "unit": "steps/s", "cores": 1, "kind": "reference",
-            "sample": "seq D (reference C sources), BRUSS2D N=2000 DOPRI54 tol 1e-6 h0 1e-3, "
-                      "first %s attempted steps (STEP_LIMIT=20)" % steps.group(1)}
-
-
-def host_info():
-    """CPU model and logical CPU count of the host the CPU baselines ran on."""
-    model = ""
-    try:
-        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
-        m = re.search(r"^Model name:\s*(.+)$", out, re.M)
-        model = m.group(1).strip() if m else ""
-    except (OSError, subprocess.SubprocessError):
-        pass
-    return {"cpu_model": model, "logical_cpus": os.cpu_count()}
+def gpu_c1():
+    """Config 1 on the GPU for the CPU comparison: N=128, t in [0, 4], DOPRI54
+    tol 1e-6, h0 1e-3 (336 attempted steps), fused kernel, inputs resident."""
+    import torch
+    import rk_amd
+    p = rk_amd.Bruss2D(128, 2e-3)
+    y0 = torch.from_numpy(p.ode_start()).cuda()
+    y = torch.empty_like(y0)
+    torch.cuda.synchronize()
+    rk_amd.solve(p, "DOPRI54", 0.0, 4.0, y0, y, 1e-6, h0=1e-3)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    st, _ = rk_amd.solve(p, "DOPRI54", 0.0, 4.0, y0, y, 1e-6, h0=1e-3)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    return {"steps": st.steps, "kernel_s": st.kernel_time, "wall_s": wall}
 
 
 def d_form_passes(s):
@@ -231,32 +272,45 @@ def ensemble_leg():
                        "plain_solves_per_s": len(chains) / t_p}}
 
 
-def serving_leg(rate_hz=4.0, duration_s=60.0, hbm_budget=4 << 30):
+def serving_leg(rate_hz=4.0, duration_s=60.0, hbm_budget=4 << 30, sweep=(2.0, 4.0)):
     """Poisson stream (seed 2) at rate_hz over duration_s: 70 % BRUSS2D N
     uniform 64..1024, 30 % LV chains M uniform 1e4..5e5, tol from {1e-4,
     1e-6, 1e-8}, t_end uniform [0.5, 4], DOPRI54, h0 1e-3.  Served in real
     time by the continuous-batching server under an HBM budget, and by the
     plain solver one request at a time (each solve timed, queueing from the
-    arrival times)."""
+    arrival times).  The sweep replays the same requests arriving `factor`
+    times as fast (8 and 16 req/s at the default rate): the batched server
+    runs again in real time; the plain server's latencies follow from its
+    measured solve times, which do not depend on arrivals when requests run
+    one at a time."""
     from rk_amd.problems import Bruss2D, LVChain
-    from rk_amd.serve import Request, serve_batched, serve_plain
+    from rk_amd.serve import Request, replay_plain, serve_batched, serve_plain
     from rk_amd.workloads import serving_stream
     stream = serving_stream(rate_hz, duration_s, seed=2)
 
-    def requests():
+    def requests(scale=1.0):
         out = []
         for k, (t, kind, size, tol, te) in enumerate(stream):
             p = Bruss2D(size, 2e-3) if kind == "bruss2d" else LVChain(size, diff=1e-3)
             y0 = p.ode_start_seeded(2, k, 0.01 if kind == "bruss2d" else 0.0)
-            out.append(Request(k, t, p, "DOPRI54", 0.0, te, y0, tol, h0=1e-3, kind=kind))
+            out.append(Request(k, t / scale, p, "DOPRI54", 0.0, te, y0, tol, h0=1e-3, kind=kind))
         return out
     _, sb = serve_batched(requests(), hbm_budget=hbm_budget)
-    _, sp = serve_plain(requests())
+    pdone, sp = serve_plain(requests())
+    rates = [{"rate_hz": rate_hz, "batched": sb, "plain": sp}]
+    for f in sweep:
+        _, sbf = serve_batched(requests(f), hbm_budget=hbm_budget)
+        rates.append({"rate_hz": rate_hz * f, "batched": sbf, "plain": replay_plain(pdone, f)})
+        print("serving at %.0f req/s: batched %.2f solves/s p99 %.2f s; plain %.2f solves/s "
+              "p99 %.2f s" % (rate_hz * f, sbf["solves_per_s"], sbf["p99_latency_s"],
+                              rates[-1]["plain"]["solves_per_s"],
+                              rates[-1]["plain"]["p99_latency_s"]), file=sys.stderr, flush=True)
     return {"workload": "Poisson %.1f req/s over %.0f s (seed 2): 70%% BRUSS2D N 64..1024, "
                         "30%% LV chain M 1e4..5e5, tol {1e-4,1e-6,1e-8}, t_end U[0.5,4]"
                         % (rate_hz, duration_s),
             "hbm_budget_bytes": hbm_budget, "batched": sb, "plain": sp,
-            "p99_latency_ratio_plain_over_batched": sp["p99_latency_s"] / sb["p99_latency_s"]}
+            "p99_latency_ratio_plain_over_batched": sp["p99_latency_s"] / sb["p99_latency_s"],
+            "rate_sweep": rates}
 
 
 def main():
@@ -368,9 +422,12 @@ def main():
     }
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        res["cpu_baseline"] = cpu_baseline(threads)
-        res["cpu_single_thread"] = cpu_single_thread()
         res["host"] = host_info()
+        leg = cpu_leg(threads)
+        res["cpu_baseline"] = cpu_baseline(leg, threads)
+        res["cpu_reference"] = leg
+        res["c1_gpu"] = gpu_c1()
+        print("cpu legs done", file=sys.stderr, flush=True)
     if not a.no_batch and a.step_limit == 0:
         res["c2_matrix"] = c2_matrix_leg()
         print("c2 matrix done", file=sys.stderr, flush=True)

@@ -211,6 +211,12 @@ int rkx_batch_remove(rkx_batch *b, int id);
 /* Device bytes held by the batch's instances. */
 size_t rkx_batch_bytes(const rkx_batch *b);
 
+/* The row-sweep tile an instance runs with now: strip width in wavefronts
+ * per component and band height in rows (both 0 for instances without the
+ * row sweep: LV chains, full storage).  Band heights shrink as the batch
+ * drains, when fewer than two tiles per compute unit would remain. */
+int rkx_batch_tile(rkx_batch *b, int id, int *waves, int *rows);
+
 /* ---- ensemble of independent LV pairs (no reference counterpart; the
  * reference's LV has one shared h for identical pairs, lv.c:31-37) ---------
  * `count` systems x' = c0 x - d0 x y, y' = d1 x y - c1 y, constants

@@ -50,6 +50,11 @@ CONFIGS = {
         dict(BRUSS_GRID_SIZE=2000, T_END="1.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=200, METHOD="DOPRI54")),
     "pthreads_bruss_D_n2000_dopri54_sl200": ("pthreads", "bruss2d-mix", "D",
         dict(BRUSS_GRID_SIZE=2000, T_END="1.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=200, METHOD="DOPRI54")),
+    # C1 in full on the host's cores (CPU baseline beside the GPU's C1 solve)
+    "pthreads_bruss_PipeD4_n128_dopri54_tol1e-6": ("pthreads", "bruss2d-mix", "PipeD4",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "pthreads_bruss_D_n128_dopri54_tol1e-6": ("pthreads", "bruss2d-mix", "D",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
     # single-thread baseline: seq D, 20-step cap (about 10 s)
     "seq_bruss_D_n2000_dopri54_sl20": ("seq", "bruss2d-mix", "D",
         dict(BRUSS_GRID_SIZE=2000, T_END="1.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=20, METHOD="DOPRI54")),

@@ -83,6 +83,7 @@ SIGNATURES = [
     ("rkx_batch_result", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("rkx_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
     ("rkx_batch_bytes", C.c_size_t, [C.c_void_p]),
+    ("rkx_batch_tile", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("rkx_lv_ensemble", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                   C.c_double, C.POINTER(Method), C.c_double, C.c_double, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),

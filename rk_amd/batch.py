@@ -91,6 +91,13 @@ class Batch:
         check(lib().rkx_batch_result(self._h, i, y.ctypes.data_as(C.POINTER(C.c_double))))
         return y
 
+    def tile(self, i):
+        """(strip waves, band rows) of instance i's row-sweep tile now ((0, 0)
+        without the row sweep)."""
+        w, r = C.c_int(), C.c_int()
+        check(lib().rkx_batch_tile(self._h, i, C.byref(w), C.byref(r)))
+        return w.value, r.value
+
     def remove(self, i):
         check(lib().rkx_batch_remove(self._h, i))
         self._n.pop(i, None)

@@ -605,4 +605,18 @@ int rkx_batch_remove(rkx_batch *b, int id)
 
 size_t rkx_batch_bytes(const rkx_batch *b) { return b ? b->bytes : 0; }
 
+int rkx_batch_tile(rkx_batch *b, int id, int *waves, int *rows)
+{
+  if (!b || !waves || !rows || id < 0 || id >= kBatchCap || !b->inst[static_cast<size_t>(id)].used)
+    return fail(RKX_EINVAL, "rkx_batch_tile: unknown instance");
+  const batch_inst &I = b->inst[static_cast<size_t>(id)];
+  *waves = 0;
+  *rows = 0;
+  if (I.storage == RKX_STORAGE_LOW && I.K != 0) {
+    *waves = I.cfg.waves;
+    *rows = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).R;
+  }
+  return RKX_OK;
+}
+
 }  // extern "C"

@@ -26,6 +26,28 @@ using gmut = __attribute__((address_space(1))) double;
 __device__ __forceinline__ gconst *as_global(const double *p) { return (gconst *) p; }
 __device__ __forceinline__ gmut *as_global(double *p) { return (gmut *) p; }
 
+// Stores v to row[idx] in the lanes where keep holds, without a branch: the
+// store goes through a buffer resource of the row (len doubles); the other
+// lanes address past its end, and the hardware drops out-of-range buffer
+// stores.  row must be wave-uniform (the resource lives in scalar registers).
+// A branch around the store would split the surrounding straight-line code
+// into scheduling regions the compiler cannot interleave.
+using u32x2 = unsigned int __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_row_masked(gmut *row, int len, int idx, bool keep, double v)
+{
+  constexpr int kGfx9RawBuffer = 0x00020000;   // dword 3 of a raw gfx9 buffer resource
+  // the row base in scalar registers (it is wave-uniform; said explicitly,
+  // the resource needs no per-lane loop)
+  const unsigned long long a = reinterpret_cast<unsigned long long>(row);
+  const unsigned int lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(a));
+  const unsigned int hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(a >> 32));
+  void *base = reinterpret_cast<void *>((static_cast<unsigned long long>(hi) << 32) | lo);
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+      base, 0, len * static_cast<int>(sizeof(double)), kGfx9RawBuffer);
+  const int off = keep ? idx * static_cast<int>(sizeof(double)) : 0x7ffffff0;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+}
+
 // A value that is the same in every lane (a step coefficient), moved to
 // scalar registers: loaded through a generic pointer it would otherwise
 // occupy a vector register pair in every lane for the whole kernel.

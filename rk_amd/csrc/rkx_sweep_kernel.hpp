@@ -280,15 +280,24 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
             // y_new = y + dy; the error term against y_old (block-inline.h:298-320)
             const double ynw = yold + acc[sl][0];
             const bool out = out_col && (kl >= r0) && (!P::full_trips || kl < r1);
-            if (out)
-              yn[static_cast<unsigned>(kl) * static_cast<unsigned>(N2) + 2 * col + VAR] = ynw;
-            // without branches: q = |e| / max(|y_new|, |y_old|); a zero
-            // divisor raises the sentinel, NaN quotients are skipped by fmax
+            // Straight-line code, so the iteration stays one scheduling
+            // region: the store is masked by buffer range checking, and the
+            // division runs in every lane on selected operands.  q = |e| /
+            // max(|y_new|, |y_old|); a zero divisor raises the sentinel; lanes
+            // not stored contribute 0/1 (qmax >= 0 already); NaN quotients
+            // are skipped by fmax.
+            store_row_masked(yn + static_cast<unsigned>(kl) * static_cast<unsigned>(N2), N2,
+                             2 * col + VAR, out, ynw);
             const double dv = fmax(fabs(ynw), fabs(yold));
-            const double qv = fabs(acc[sl][s]) / dv;
             const bool dz = (dv == 0.0);
+            const bool take = out && !dz;
+            double num = take ? fabs(acc[sl][s]) : 0.0, den = take ? dv : 1.0;
+            // opaque operands: otherwise the compiler folds 0/1 and puts the
+            // division back under a branch
+            __asm__("" : "+v"(num), "+v"(den));
+            const double qv = num / den;
             zero |= (out && dz) ? 1u : 0u;
-            qmax = (out && !dz) ? fmax(qmax, qv) : qmax;
+            qmax = fmax(qmax, qv);
           }
         });
       });

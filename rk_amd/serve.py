@@ -112,6 +112,26 @@ def serve_batched(requests, hbm_budget=64 << 30, poll_steps=64, realtime=True, c
     return done, summ
 
 
+def replay_plain(reqs, scale=1.0):
+    """The plain server's latencies for the same requests arriving `scale`
+    times as fast, from the solve durations serve_plain measured: the plain
+    server runs one request at a time, so a request's solve time does not
+    depend on when it arrived, only its start does (max of its arrival and the
+    previous completion)."""
+    clock = 0.0
+    out = []
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        a = r.arrival / scale
+        start = max(clock, a)
+        clock = start + (r.finished - r.admitted)
+        out.append(clock - a)
+    s = {"requests": len(out), "makespan_s": clock,
+         "solves_per_s": len(out) / clock if clock > 0 else float("nan"),
+         "p50_latency_s": percentile(out, 50), "p99_latency_s": percentile(out, 99),
+         "mean_latency_s": float(np.mean(out)) if out else float("nan")}
+    return s
+
+
 def serve_plain(requests, realtime_model=True, keep_results=False):
     """The comparator: each request through the plain solver (rk_amd.solve),
     one at a time in arrival order.  A request starts at max(its arrival,

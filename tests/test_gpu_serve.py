@@ -54,6 +54,28 @@ def test_served_results_equal_plain(rk, realtime, budget):
         assert r.admitted >= (r.arrival if realtime else 0.0) and r.finished >= r.admitted
 
 
+def test_served_results_equal_oracle(rk, orc):
+    """Served requests against the CPU oracle (seq D restated), not only
+    against the plain GPU solver: the requests of the stream above that the
+    oracle finishes in seconds, each bitwise with its step counts."""
+    from rk_amd.serve import serve_batched
+    done, _ = serve_batched(stream(rk), hbm_budget=64 << 30, poll_steps=16, realtime=False,
+                            keep_results=True)
+    checked = 0
+    for r in done:
+        if r.problem.n > 40_000:
+            continue
+        if r.kind == "bruss2d":
+            po = orc.make_ode("bruss2d", r.problem.N, r.problem.alpha)
+        else:
+            po = orc.make_ode("lvchain", r.problem.cells, diff=r.problem.diff)
+        yo, so, _ = orc.solve(po, r.method, r.t0, r.te, r.y0, r.tol, h0=r.h0)
+        assert r.steps == so["steps"], r.rid
+        assert np.array_equal(r.result, yo), r.rid
+        checked += 1
+    assert checked >= 4
+
+
 def test_budget_limits_resident_set(rk):
     """An 800 kB budget holds only a few of the requests at a time (about 1.7
     MB in all): the server queues the rest and admits them as earlier ones

@@ -130,6 +130,40 @@ def test_lv_invariant(orc):
     assert abs(V(y) - V(y0)) < 1e-8
 
 
+# four distinct constants (the defaults have d0 == c1 == 0.2, so a swap of two
+# of them would pass the checks above unnoticed)
+LV_DISTINCT = (0.37, 1.21, 0.53, 1.77)
+
+
+def test_lv_rhs_closed_form_distinct_constants(orc):
+    c0, d0, d1, c1 = LV_DISTINCT
+    p = orc.make_ode("lv", 5, lv=LV_DISTINCT)
+    rng = np.random.default_rng(3)
+    y = rng.uniform(0.1, 2.0, p.n)
+    f = orc.eval_rng(p, y)
+    x, z = y[0::2], y[1::2]
+    assert np.array_equal(f[0::2], c0 * x - d0 * x * z)
+    assert np.array_equal(f[1::2], d1 * x * z - c1 * z)
+    for i in range(p.n):
+        assert orc.eval_comp(p, y, i) == f[i]
+
+
+def test_lv_invariant_distinct_constants(orc):
+    """V = d1 x - c1 ln x + d0 y - c0 ln y is conserved for x' = c0 x - d0 x y,
+    y' = d1 x y - c1 y; with any two constants swapped it is not."""
+    c0, d0, d1, c1 = LV_DISTINCT
+    p = orc.make_ode("lv", 1, lv=LV_DISTINCT)
+    y0 = np.array([0.8, 0.45])
+    y, st, _ = orc.solve(p, "VERNER65", 0.0, 15.0, y0, 1e-11, h0=1e-3)
+
+    def V(v, a, b, c, d):
+        x, z = v
+        return c * x - d * math.log(x) + b * z - a * math.log(z)
+    assert abs(V(y, c0, d0, d1, c1) - V(y0, c0, d0, d1, c1)) < 1e-8
+    # the same check with d0 and c1 exchanged fails: the pin sees the swap
+    assert abs(V(y, c0, c1, d1, d0) - V(y0, c0, c1, d1, d0)) > 1e-3
+
+
 def test_chain_zero_diffusion_is_pairs(orc):
     """The LV chain with D = 0 is the independent LV pairs, bitwise."""
     pc = orc.make_ode("lvchain", 16, diff=0.0)
