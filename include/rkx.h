@@ -242,6 +242,24 @@ int rkx_pow_device(const double *x_dev, const double *y_dev, double *p_dev, size
                    void *stream);
 int rkx_pow_host(const double *x, const double *y, double *p, size_t n);
 
+/* ---- solve without allocation ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid allocation): rkx_solve
+ * for the fused low-storage forms (impl RKX_IMPL_PIPEDLS or RKX_IMPL_PIPED;
+ * BRUSS2D and LV chains with a table that has a fused kernel), the same
+ * arguments and bitwise the same results.  y_dev serves as one of the two y
+ * buffers; the other and the step state live in a 32 MiB device array the
+ * library holds on each device it runs on until the process ends.  Makes no
+ * device or pinned allocation and creates no graph or event; with o->stream
+ * set it also uses no stream of its own.  More than
+ * RKX_NOALLOC_MAX_COMPONENTS components: RKX_ELIMIT; trace or dump output,
+ * the D form, or a problem without a fused kernel: RKX_EINVAL.  Calls that
+ * overlap on one device share that array and race. */
+#define RKX_NOALLOC_MAX_COMPONENTS 4186112   /* (32 MiB - 64 KiB) / 8 */
+int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double te,
+                      const double *y0_dev, double *y_dev, double tol, const rkx_opts *o,
+                      rkx_stats *st);
+
 /* ---- runtime -------------------------------------------------------------- */
 const char *rkx_last_error(void);
 int rkx_device_sync(void);

@@ -87,6 +87,9 @@ SIGNATURES = [
     ("rkx_lv_ensemble", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                   C.c_double, C.POINTER(Method), C.c_double, C.c_double, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rkx_solve_noalloc", C.c_int, [C.POINTER(Ode), C.POINTER(Method), C.c_double, C.c_double,
+                                    C.c_void_p, C.c_void_p, C.c_double, C.POINTER(Opts),
+                                    C.POINTER(Stats)]),
     ("rkx_pow_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     ("rkx_pow_host", C.c_int, [_dp, _dp, _dp, C.c_size_t]),
     ("rkx_last_error", C.c_char_p, []),

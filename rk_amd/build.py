@@ -29,6 +29,7 @@ SOURCES = [
     "rkx_kernels_lv.hip",
     "rkx_kernels_chain.hip",
     "rkx_pow.hip",
+    "rkx_noalloc.cpp",
 ]
 
 # Reference-style solver binaries over the C ABI (rk_amd/csrc/dropin): name ->
