@@ -149,12 +149,42 @@ def d_form_passes(s):
     return (s + 2) + sum(5 + 2 * (s - 1 - i) for i in range(1, s - 1)) + 5
 
 
-def c2_matrix_leg(window=(300, 1300)):
+def gather_form_passes(meth, impl):
+    """Vector passes of one step of the gather forms.  A and E: stage 0 reads
+    y and writes v_0; stage l gathers (reads y and the v_i it adds, writes w)
+    then evaluates (reads w, writes v_l); the output reads the v_i it adds and
+    y, and writes y_new.  A adds only the non-zero coefficients of the table.
+    F: stage l < s-1 reads w_l (stage 0: y), y and v_0..v_{l-1}, and writes
+    v_l and w_{l+1}; the last stage reads w_{s-1} and writes v_{s-1}."""
+    from rk_amd.methods import method
+    m = method(meth)
+    s = m.s
+    A = [[m.A[l][i] for i in range(s)] for l in range(s)]
+    if impl == "F":
+        return 3 + sum(l + 4 for l in range(1, s - 1)) + 2 + (s + 2)
+    skip = impl == "A"
+    nnz = [sum(1 for i in range(l) if A[l][i] != 0.0 or not skip) for l in range(s)]
+    out = 1 + sum(1 for i in range(1, s)
+                  if not skip or m.b[i] != 0.0 or m.b[i] - m.b_hat[i] != 0.0)
+    return 2 + sum(2 + nnz[l] + 2 for l in range(1, s)) + out + 2
+
+
+def form_passes(meth, impl):
+    from rk_amd.methods import method
+    if impl == "D":
+        return d_form_passes(method(meth).s)
+    if impl in ("A", "E", "F"):
+        return gather_form_passes(meth, impl)
+    return 2
+
+
+def c2_matrix_leg(window=(300, 1300), impls=("D", "A", "E", "F", "PipeDls")):
     """BRUSS2D N=2000, t in [0, 1], h0 1e-3: {DOPRI54, VERNER65, and the 13-stage
-    DOPRI87} x tol {1e-6, 1e-8} x {full storage D form, low-storage fused
-    PipeDls}; each timed over attempted steps window[0]..window[1] of the
-    solve (differenced, so set-up is excluded), with the algorithmic HBM bytes
-    of the form and the device bytes the solve holds."""
+    DOPRI87} x tol {1e-6, 1e-8} x {full-storage loop orders D, A, E, F,
+    low-storage fused PipeDls}; each timed over attempted steps
+    window[0]..window[1] of the solve (differenced, so set-up is excluded),
+    with the algorithmic HBM bytes of the form and the device bytes the solve
+    holds."""
     import torch
     import rk_amd
     p = rk_amd.Bruss2D(N_GRID, 2e-3)
@@ -163,14 +193,13 @@ def c2_matrix_leg(window=(300, 1300)):
     torch.cuda.synchronize()
     rows = []
     for meth in ("DOPRI54", "VERNER65", "DOPRI87"):
-        s = rk_amd.method(meth).s
         for tol in (1e-6, 1e-8):
-            for impl in ("D", "PipeDls"):
+            for impl in impls:
                 kw = dict(h0=1e-3, impl=impl)
                 a, _ = rk_amd.solve(p, meth, 0.0, 1.0, y0, y, tol, step_limit=window[0], **kw)
                 b, _ = rk_amd.solve(p, meth, 0.0, 1.0, y0, y, tol, step_limit=window[1], **kw)
                 us = (b.kernel_time - a.kernel_time) / (b.steps - a.steps) * 1e6
-                passes = d_form_passes(s) if impl == "D" else 2
+                passes = form_passes(meth, impl)
                 rows.append({"method": meth, "tol": tol, "impl": impl, "us_per_step": us,
                              "hbm_gbs_algorithmic": passes * 8.0 * p.n / (us * 1e-6) / 1e9,
                              "vector_passes_per_step": passes,
@@ -332,6 +361,9 @@ def main():
                     help="cap each solve at this many attempted steps (profiling runs)")
     ap.add_argument("--leg", choices=("c2_matrix", "batched", "ensemble", "serving"),
                     help="run only this leg and print its JSON (for profiling runs)")
+    ap.add_argument("--grid", type=int, default=N_GRID,
+                    help="BRUSS2D grid size of the main leg (profiling runs; the metric is "
+                         "quoted at 2000)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 5 plain device copies of n doubles, the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -354,7 +386,7 @@ def main():
 
     assert torch.cuda.is_available()
     stream = torch.cuda.Stream()
-    p = rk_amd.Bruss2D(N_GRID, 2e-3)
+    p = rk_amd.Bruss2D(a.grid, 2e-3)
     y0 = torch.from_numpy(p.ode_start()).cuda()
     y = torch.empty_like(y0)
     torch.cuda.synchronize()
@@ -408,10 +440,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (BRUSS2D start vector of bruss2d-mix.c:51-59)",
-        "config": {"workload": "BRUSS2D N=2000 full solve t in [0,1], %s tol %g h0 1e-3, "
-                               "%s fused step kernel, device step control" % (a.method, a.tol,
-                                                                              a.impl),
-                   "grid": N_GRID, "n": n, "method": a.method, "tol": a.tol,
+        "config": {"workload": "BRUSS2D N=%d full solve t in [0,1], %s tol %g h0 1e-3, "
+                               "%s fused step kernel, device step control" % (a.grid, a.method,
+                                                                              a.tol, a.impl),
+                   "grid": a.grid, "n": n, "method": a.method, "tol": a.tol,
                    "attempted_steps_per_solve": stats[-1]["steps"],
                    "accepted": stats[-1]["acc"], "rejected": stats[-1]["rej"],
                    "hbm_gbs_algorithmic": achieved,

@@ -101,8 +101,14 @@ enum {
   RKX_IMPL_PIPED = 1,    /* the same stage-fused sweep as RKX_IMPL_PIPEDLS: the fused form
                             keeps every stage vector of src/impl/seq/PipeD.c in registers,
                             so it has no full-storage variant (two vectors of n) */
-  RKX_IMPL_PIPEDLS = 2   /* stage-fused sweep, low storage: two vectors of n
+  RKX_IMPL_PIPEDLS = 2,  /* stage-fused sweep, low storage: two vectors of n
                             (src/impl/seq/PipeDls.c) */
+  RKX_IMPL_A = 3,        /* gather form skipping zero coefficients: w_l = y + sum hA v_i,
+                            then v_l = f(w_l) (src/impl/seq/A.c); s + 3 vectors of n */
+  RKX_IMPL_E = 4,        /* the gather form without the zero skip (src/impl/seq/E.c) */
+  RKX_IMPL_F = 5         /* RHS and the next stage's gather in one pass, two w vectors
+                            (src/impl/seq/F.c); s + 4 vectors of n.  D, A, E and F run on
+                            every problem and give the same results bitwise */
 };
 
 enum {

@@ -60,6 +60,21 @@ CONFIGS = {
         dict(BRUSS_GRID_SIZE=2000, T_END="1.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=20, METHOD="DOPRI54")),
 }
 
+# the reference's other loop orders (src/impl/seq/A.c, E.c, F.c) on known-answer
+# cases: KAT name -> the defines of that case; binaries seq_<tag>_<impl>_<case>
+GATHER_IMPLS = ("A", "E", "F")
+GATHER_KAT = {
+    "c1_bruss128_dopri54_tol1e-6": ("bruss2d-mix", "bruss",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+    "bruss128_verner65_tol1e-8": ("bruss2d-mix", "bruss",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-8", H_INIT="1E-3", STEP_LIMIT=0, METHOD="VERNER65")),
+    "lv10000_dopri54": ("lv", "lv",
+        dict(LV_COUNT=10000, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
+}
+for kat, (prob, tag, defs) in GATHER_KAT.items():
+    for impl in GATHER_IMPLS:
+        CONFIGS["seq_%s_%s_%s" % (tag, impl, kat)] = ("seq", prob, impl, defs)
+
 # DUMP trajectories (every attempted step, full vector) on tiny grids
 DUMP_CASES = [
     ("bruss2d-mix", dict(BRUSS_GRID_SIZE=4), "DOPRI54", "4.0", "1E-6", "1E-3"),
@@ -73,11 +88,17 @@ DUMP_CASES = [
     ("lv", dict(LV_COUNT=3), "DOPRI54", "20.0", "1E-8", "1E-3"),
     ("lv", dict(LV_COUNT=2), "VERNER65", "10.0", "1E-6", "1E-2"),
 ]
+# the same trajectories from the reference's A, E and F (one case per method family)
+GATHER_DUMPS = ("dump_bruss_n4_dopri54", "dump_bruss_n5_verner65", "dump_bruss_n3_dopri87",
+                "dump_bruss_n6_rkf23", "dump_lv_n3_dopri54")
 for prob, d, meth, te, tol, h0 in DUMP_CASES:
     tag = "n%d" % d.get("BRUSS_GRID_SIZE", d.get("LV_COUNT"))
     name = "dump_%s_%s_%s" % ("bruss" if prob.startswith("bruss") else "lv", tag, meth.lower())
     defs = dict(d, T_END=te, TOL=tol, H_INIT=h0, STEP_LIMIT=0, METHOD=meth, DUMP=1)
     CONFIGS[name] = ("seq", prob, "D", defs)
+    if name in GATHER_DUMPS:
+        for impl in GATHER_IMPLS:
+            CONFIGS[name + "_" + impl] = ("seq", prob, impl, defs)
 
 
 # name -> (reference problem source, drop-in descriptor, defines); each

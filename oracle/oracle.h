@@ -11,17 +11,19 @@
  * premultiplication (src/methods.c:43-58), the method tables
  * (src/methods.c:62-724) and the two problems (src/problems/bruss2d-mix.c,
  * src/problems/lv.c).  The pipelined schedules PipeD / PipeDls
- * (src/impl/seq/PipeD.c, src/impl/seq/PipeDls.c) and the pthreads D
- * (src/impl/pthreads/D.c) are restated as well; they must agree bitwise with D.
+ * (src/impl/seq/PipeD.c, src/impl/seq/PipeDls.c), the gather forms A, E, F
+ * (src/impl/seq/A.c, E.c, F.c) and the pthreads D (src/impl/pthreads/D.c) are
+ * restated as well; they must agree bitwise with D.
  *
  * Every floating-point expression keeps the reference's operand order; the
  * library is compiled with -O3 -ffp-contract=off and without -march, so no
  * multiply-add is ever fused (the reference relies on baseline x86-64).
  *
  * Pinning (how each part is checked, see tests/test_oracle_*.py):
- *   - orc_solve_D / PipeD / PipeDls: known-answer footprints and step counts
- *     produced by the compiled reference, and DUMP trajectories (every
- *     attempted step, full vectors) at tiny grids, bitwise.
+ *   - orc_solve_D / PipeD / PipeDls / A / E / F: known-answer footprints and
+ *     step counts produced by the compiled reference, and DUMP trajectories
+ *     (every attempted step, full vectors) at tiny grids, bitwise; the
+ *     reference's own A, E and F binaries reproduce D's trajectories.
  *   - BRUSS2D RHS: brute-force dense Neumann Laplacian on tiny grids; the
  *     homogeneous steady state u=1, v=3.4.
  *   - LV RHS: closed form; the conserved quantity of the pair.
@@ -113,6 +115,13 @@ int orc_solve_PipeD(const orc_ode *p, const orc_method *m, double t0, double te,
 int orc_solve_PipeDls(const orc_ode *p, const orc_method *m, double t0, double te,
                       const double *y0, double *y, double tol, const orc_opts *o,
                       orc_stats *st);
+/* gather forms (src/impl/seq/A.c, E.c, F.c) */
+int orc_solve_A(const orc_ode *p, const orc_method *m, double t0, double te, const double *y0,
+                double *y, double tol, const orc_opts *o, orc_stats *st);
+int orc_solve_E(const orc_ode *p, const orc_method *m, double t0, double te, const double *y0,
+                double *y, double tol, const orc_opts *o, orc_stats *st);
+int orc_solve_F(const orc_ode *p, const orc_method *m, double t0, double te, const double *y0,
+                double *y, double tol, const orc_opts *o, orc_stats *st);
 int orc_solve_pthreads_D(const orc_ode *p, const orc_method *m, double t0, double te,
                          const double *y0, double *y, double tol, const orc_opts *o,
                          int threads, orc_stats *st);

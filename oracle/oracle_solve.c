@@ -1,6 +1,7 @@
 /*
  * Oracle solvers: seq D (the parity reference), the pipelined PipeD and
- * low-storage PipeDls schedules, and pthreads D.  TEST INFRASTRUCTURE ONLY.
+ * low-storage PipeDls schedules, the gather forms A, E and F, and pthreads D.
+ * TEST INFRASTRUCTURE ONLY.
  */
 
 #include "oracle.h"
@@ -377,6 +378,137 @@ int orc_solve_PipeDls(const orc_ode *p, const orc_method *m, double t0, double t
                       orc_stats *st)
 {
   return solve_pipe(p, m, t0, te, y0, y, tol, o, st, 1);
+}
+
+/* ---- gather forms A, E, F (src/impl/seq/A.c:23-103, E.c:23-101, F.c:23-106)
+ * Stage vectors v_0..v_{s-1} are kept whole.  A and E evaluate v_0 = f(y),
+ * then per stage gather w = y + sum_{i<l} hA[l][i] v_i (A: non-zero entries
+ * only, block-inline.h:76-92; E: every entry, :38-50) and v_l = f(w).  F
+ * evaluates v_l = f(w_l) and gathers w_{l+1} in the same loop over j
+ * (block_rhs_gather_interm_stage, :165-182, w_0 = y), the last stage on its
+ * own.  The output gathers err and dy (A: block_gather_vec_output, :96-134;
+ * E and F: block_gather_output, :54-72) and updates y (A.c:85-92). */
+
+enum { GATHER_A, GATHER_E, GATHER_F };
+
+static int solve_gather(const orc_ode *p, const orc_method *m, double t0, double te,
+                        const double *y0, double *y, double tol, const orc_opts *o,
+                        orc_stats *st, int form)
+{
+  if (check_args(p, m))
+    return -1;
+  const int n = p->n, s = m->s;
+  const int limit = o ? o->step_limit : 0;
+  const int skip = (form == GATHER_A);
+  coef_t k;
+  coef_init(&k, m);
+  double *v[ORC_MAX_S];
+  for (int l = 0; l < s; l++)
+    v[l] = (double *) malloc((size_t) n * sizeof(double));
+  double *wa = (double *) malloc((size_t) n * sizeof(double));
+  double *wb = (double *) malloc((size_t) n * sizeof(double));
+  double *err = (double *) malloc((size_t) n * sizeof(double));
+  double *dy = (double *) malloc((size_t) n * sizeof(double));
+  int acc = 0, rej = 0;
+  double h = orc_initial_stepsize(te - t0, tol, limit, o ? o->h0 : 0.0);
+  double t;
+  memcpy(y, y0, (size_t) n * sizeof(double));
+  const double t_start = now_s();
+  for (t = t0; keep_going(t, te, acc, rej, limit); h = fmin(h, te - t)) {
+    premult(&k, m, h);
+    if (form == GATHER_F) {
+      double *wbuf[2] = {wa, wb};  /* w_cur / w_next, swapped per stage */
+      const double *wl = y;        /* w_0 = y */
+      for (int l = 0; l < s - 1; l++) {
+        double *wn = wbuf[l & 1];
+        for (int j = 0; j < n; j++) {
+          v[l][j] = orc_eval_comp(p, j, t + k.hc[l], wl);
+          double a = y[j];
+          for (int i = 0; i <= l; i++)
+            a += k.hA[l + 1][i] * v[i][j];
+          wn[j] = a;
+        }
+        wl = wn;
+      }
+      orc_eval_rng(p, 0, n - 1, t + k.hc[s - 1], wl, v[s - 1]);
+    } else {
+      orc_eval_rng(p, 0, n - 1, t + k.hc[0], y, v[0]);
+      for (int l = 1; l < s; l++) {
+        if (skip) {
+          if (k.nzA[l][0])
+            for (int j = 0; j < n; j++)
+              wa[j] = y[j] + k.hA[l][0] * v[0][j];
+          else
+            memcpy(wa, y, (size_t) n * sizeof(double));
+          for (int i = 1; i < l; i++)
+            if (k.nzA[l][i])
+              for (int j = 0; j < n; j++)
+                wa[j] += k.hA[l][i] * v[i][j];
+        } else {
+          for (int j = 0; j < n; j++) {
+            double a = y[j];
+            for (int i = 0; i < l; i++)
+              a += k.hA[l][i] * v[i][j];
+            wa[j] = a;
+          }
+        }
+        orc_eval_rng(p, 0, n - 1, t + k.hc[l], wa, v[l]);
+      }
+    }
+    for (int j = 0; j < n; j++) {
+      err[j] = k.he[0] * v[0][j];
+      dy[j] = k.hb[0] * v[0][j];
+    }
+    for (int i = 1; i < s; i++) {
+      const int te_ = !skip || k.nze[i], tb = !skip || k.nzb[i];
+      for (int j = 0; j < n; j++) {
+        if (te_)
+          err[j] += k.he[i] * v[i][j];
+        if (tb)
+          dy[j] += k.hb[i] * v[i][j];
+      }
+    }
+    double err_max = 0.0;
+    for (int j = 0; j < n; j++) {
+      const double yj_old = y[j];
+      y[j] += dy[j];
+      dy[j] = yj_old;              /* dy now holds y_old (A.c:85-90) */
+      err_max = orc_update_error_max(err_max, err[j], y[j], yj_old);
+    }
+    step_control(&t, &h, err_max, m->ord, tol, y, dy, n, &acc, &rej, o);
+  }
+  if (st) {
+    st->steps_acc = acc;
+    st->steps_rej = rej;
+    st->t = t;
+    st->h = h;
+    st->kernel_time = now_s() - t_start;
+  }
+  for (int l = 0; l < s; l++)
+    free(v[l]);
+  free(wa);
+  free(wb);
+  free(err);
+  free(dy);
+  return 0;
+}
+
+int orc_solve_A(const orc_ode *p, const orc_method *m, double t0, double te, const double *y0,
+                double *y, double tol, const orc_opts *o, orc_stats *st)
+{
+  return solve_gather(p, m, t0, te, y0, y, tol, o, st, GATHER_A);
+}
+
+int orc_solve_E(const orc_ode *p, const orc_method *m, double t0, double te, const double *y0,
+                double *y, double tol, const orc_opts *o, orc_stats *st)
+{
+  return solve_gather(p, m, t0, te, y0, y, tol, o, st, GATHER_E);
+}
+
+int orc_solve_F(const orc_ode *p, const orc_method *m, double t0, double te, const double *y0,
+                double *y, double tol, const orc_opts *o, orc_stats *st)
+{
+  return solve_gather(p, m, t0, te, y0, y, tol, o, st, GATHER_F);
 }
 
 /* ---- pthreads D (src/impl/pthreads/D.c:50-218) --------------------------- */

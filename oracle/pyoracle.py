@@ -73,7 +73,8 @@ def lib():
         L.orc_initial_stepsize.restype = C.c_double
         sig = [C.POINTER(Ode), C.POINTER(Method), C.c_double, C.c_double, dp, dp, C.c_double,
                C.POINTER(Opts)]
-        for fn in ("orc_solve_D", "orc_solve_PipeD", "orc_solve_PipeDls"):
+        for fn in ("orc_solve_D", "orc_solve_PipeD", "orc_solve_PipeDls", "orc_solve_A",
+                   "orc_solve_E", "orc_solve_F"):
             getattr(L, fn).argtypes = sig + [C.POINTER(Stats)]
         L.orc_solve_pthreads_D.argtypes = sig + [C.c_int, C.POINTER(Stats)]
         ip = C.POINTER(C.c_int)

@@ -12,8 +12,8 @@ There is no CPU fallback: importing a function that needs the library raises
 if librkx.so is missing or no HIP device is present.
 """
 from ._lib import (RKX_BRUSS2D, RKX_LV, RKX_LV_CHAIN, RKX_IMPL_D, RKX_IMPL_PIPED,
-                   RKX_IMPL_PIPEDLS, RKX_CTRL_DEVICE, RKX_CTRL_HOST, RkxError, lib,
-                   lib_path)
+                   RKX_IMPL_PIPEDLS, RKX_IMPL_A, RKX_IMPL_E, RKX_IMPL_F, RKX_CTRL_DEVICE,
+                   RKX_CTRL_HOST, RkxError, lib, lib_path)
 from .problems import Bruss2D, LV, LVChain, Problem
 from .methods import method, method_table, METHODS
 from .solver import solve, solve_host, solver, Stats
@@ -21,7 +21,7 @@ from .batch import Batch
 
 __all__ = [
     "RKX_BRUSS2D", "RKX_LV", "RKX_LV_CHAIN", "RKX_IMPL_D", "RKX_IMPL_PIPED",
-    "RKX_IMPL_PIPEDLS", "RKX_CTRL_DEVICE", "RKX_CTRL_HOST", "RkxError", "lib", "lib_path",
+    "RKX_IMPL_PIPEDLS", "RKX_IMPL_A", "RKX_IMPL_E", "RKX_IMPL_F", "RKX_CTRL_DEVICE", "RKX_CTRL_HOST", "RkxError", "lib", "lib_path",
     "Bruss2D", "LV", "LVChain", "Problem", "method", "method_table", "METHODS",
     "solve", "solve_host", "solver", "Stats", "Batch",
 ]

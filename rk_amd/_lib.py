@@ -8,6 +8,7 @@ _LIB_FILE = os.path.join(HERE, "librkx.so")
 RKX_MAX_STAGES = 16
 RKX_BRUSS2D, RKX_LV, RKX_LV_CHAIN = 1, 2, 3
 RKX_IMPL_D, RKX_IMPL_PIPED, RKX_IMPL_PIPEDLS = 0, 1, 2
+RKX_IMPL_A, RKX_IMPL_E, RKX_IMPL_F = 3, 4, 5
 RKX_CTRL_DEVICE, RKX_CTRL_HOST = 0, 1
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ELIMIT"}

@@ -1,4 +1,5 @@
-"""Tile-shape autotuner for batches of fused BRUSS2D instances.
+"""Autotuners: the tile shape for batches of fused BRUSS2D instances, and the
+step form (loop order) of a single solve.
 
 The fused kernel's strip width (1..4 wavefronts per component: 64..256 grid
 columns per workgroup) and band height (rows per workgroup) trade wasted
@@ -42,3 +43,29 @@ def batch_tiles(instances, meth="DOPRI54", t0=0.0, te=4.0, tol=1e-6, h0=1e-3, sa
             times[(K, R)] = time.perf_counter() - t
     best = min(times, key=times.get)
     return best, times
+
+
+FORMS = ("PipeDls", "D", "A", "E", "F")
+
+
+def solve_form(problem, meth, y0, tol, t0=0.0, te=1.0, h0=1e-3, window=(20, 120), forms=FORMS):
+    """Time the step forms on one problem: the fused low-storage sweep
+    (PipeDls; a problem without a fused kernel runs the D stage kernels under
+    that name) and the full-storage loop orders D, A, E and F of the
+    reference (src/impl/seq/D.c, A.c, E.c, F.c).  All of them give the same
+    steps and results bitwise, so per-step times compare directly; each is
+    the difference of two solves capped at window[0] and window[1] attempted
+    steps (set-up excluded).  y0: device tensor.  Returns (fastest form,
+    {form: seconds per attempted step})."""
+    import torch
+    from .solver import solve
+    y = torch.empty_like(y0)
+    times = {}
+    for f in forms:
+        a, _ = solve(problem, meth, t0, te, y0, y, tol, h0=h0, impl=f, step_limit=window[0])
+        b, _ = solve(problem, meth, t0, te, y0, y, tol, h0=h0, impl=f, step_limit=window[1])
+        if b.steps > a.steps:
+            times[f] = (b.kernel_time - a.kernel_time) / (b.steps - a.steps)
+    if not times:
+        raise ValueError("solve_form: the solve ends before window[0] attempted steps")
+    return min(times, key=times.get), times

@@ -24,6 +24,7 @@ SOURCES = [
     "rkx_batch.cpp",
     "rkx_methods.cpp",
     "rkx_kernels_d.hip",
+    "rkx_kernels_gather.hip",
     "rkx_kernels_sweep.hip",
     "rkx_kernels_sweep_dopri87.hip",
     "rkx_kernels_lv.hip",

@@ -17,11 +17,21 @@
 
 static const char *variant_name(int impl)
 {
-  return impl == RKX_IMPL_D ? "hipD (stage kernels of D on the device)"
-       : impl == RKX_IMPL_PIPED ? "hipPipeD (stage-fused temporally blocked sweep, "
-                                  "two vectors resident; the PipeDls kernel)"
-                                : "hipPipeDls (stage-fused temporally blocked sweep, "
-                                  "two vectors resident)";
+  switch (impl) {
+  case RKX_IMPL_D:
+    return "hipD (stage kernels of D on the device)";
+  case RKX_IMPL_PIPED:
+    return "hipPipeD (stage-fused temporally blocked sweep, two vectors resident; "
+           "the PipeDls kernel)";
+  case RKX_IMPL_A:
+    return "hipA (gather stage kernels skipping zero coefficients on the device)";
+  case RKX_IMPL_E:
+    return "hipE (gather stage kernels on the device)";
+  case RKX_IMPL_F:
+    return "hipF (fused RHS and gather stage kernels on the device)";
+  default:
+    return "hipPipeDls (stage-fused temporally blocked sweep, two vectors resident)";
+  }
 }
 
 void solver(double t0, double te, double *y0, double *y, double tol)

@@ -422,8 +422,13 @@ size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
   const size_t fixed = sizeof(state) + sizeof(dev_method) + sizeof(dev_coef);
   const dev_ode dp = make_dev_ode(*p);
   const bool fused = sweep_supported(dp, *m) || chain_supported(dp, *m);
+  const size_t s = static_cast<size_t>(m->s);
+  if (impl == RKX_IMPL_A || impl == RKX_IMPL_E)
+    return (2 + s + 1) * vec + fixed;            // y x2, v_0..v_{s-1}, w
+  if (impl == RKX_IMPL_F)
+    return (2 + s + 2) * vec + fixed;            // y x2, v_0..v_{s-1}, w x2
   if (impl == RKX_IMPL_D || !fused)
-    return (2 + (m->s - 1) + 2) * vec + fixed;   // y x2, w_1..w_{s-1}, dy, err
+    return (2 + (s - 1) + 2) * vec + fixed;      // y x2, w_1..w_{s-1}, dy, err
   return 2 * vec + fixed;                        // y x2
 }
 
@@ -442,7 +447,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     o = *o_in;
   else
     rkx_opts_default(&o);
-  if (o.impl < RKX_IMPL_D || o.impl > RKX_IMPL_PIPEDLS)
+  if (o.impl < RKX_IMPL_D || o.impl > RKX_IMPL_F)
     return fail(RKX_EINVAL, "rkx_solve: unknown impl");
   if (o.ctrl != RKX_CTRL_DEVICE && o.ctrl != RKX_CTRL_HOST)
     return fail(RKX_EINVAL, "rkx_solve: unknown step-control mode");
@@ -460,16 +465,22 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   int impl = o.impl;
   // fused forms: the row sweep for BRUSS2D, the segment kernel for the LV
   // chain; anything else (LV pairs, tables without a compiled pattern) runs
-  // the stage kernels of D
-  const bool chain = impl != RKX_IMPL_D && chain_supported(dp, *m);
-  if (impl != RKX_IMPL_D && !chain && !sweep_supported(dp, *m))
+  // the stage kernels of D.  A, E and F run as asked on every problem.
+  const bool gather = impl == RKX_IMPL_A || impl == RKX_IMPL_E || impl == RKX_IMPL_F;
+  bool fused = impl == RKX_IMPL_PIPED || impl == RKX_IMPL_PIPEDLS;
+  const bool chain = fused && chain_supported(dp, *m);
+  if (fused && !chain && !sweep_supported(dp, *m)) {
     impl = RKX_IMPL_D;
+    fused = false;
+  }
+  const bool sweep = fused && !chain;
   const size_t n = static_cast<size_t>(p->n);
   const size_t vec = n * sizeof(double);
 
   solve_mem mem;
   double *ybuf[2] = {nullptr, nullptr};
   double *w = nullptr, *dy = nullptr, *err = nullptr, *trace_dev = nullptr;
+  double *v = nullptr, *w1 = nullptr;   // gather forms: v_0..v_{s-1}; F's second w
   state *st_dev = nullptr;
   dev_method *dm_dev = nullptr;
   dev_coef *cf_dev = nullptr;
@@ -489,7 +500,16 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
       return rc;
     if (int rc = mem.alloc(reinterpret_cast<void **>(&err), vec))
       return rc;
+  } else if (gather) {
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&v), vec * s))
+      return rc;
+    if (int rc = mem.alloc(reinterpret_cast<void **>(&w), vec))
+      return rc;
+    if (impl == RKX_IMPL_F)
+      if (int rc = mem.alloc(reinterpret_cast<void **>(&w1), vec))
+        return rc;
   }
+  const gather_masks nz = make_gather_masks(*m);
   if (o.trace_cap > 0)
     if (int rc = mem.alloc(reinterpret_cast<void **>(&trace_dev),
                            sizeof(double) * 3 * static_cast<size_t>(o.trace_cap)))
@@ -520,7 +540,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   RKX_HIP(hipMemcpyAsync(ybuf[0], y0_dev, vec, hipMemcpyDeviceToDevice, stream));
 
   sweep_cfg cfg{1, 1};
-  if (impl != RKX_IMPL_D && !chain)
+  if (sweep)
     cfg = sweep_default_cfg(dp, *m);
   if (o.tile_rows > 0)
     cfg.rows = o.tile_rows;
@@ -533,7 +553,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   chain_inst *chain_dev = nullptr;
   int ntiles = 0;
   const int pattern = sweep_pattern(*m);
-  if (impl != RKX_IMPL_D && !chain && sweep_resident_blocks(pattern, cfg.waves) < 1)
+  if (sweep && sweep_resident_blocks(pattern, cfg.waves) < 1)
     return fail(RKX_EINVAL, "rkx_solve: tile_waves wider than this table's fused kernel "
                             "(DOPRI87: at most 2)");
   if (chain) {
@@ -543,7 +563,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     ntiles = hi.nwg;
     RKX_HIP(hipMemcpyAsync(chain_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));
     RKX_HIP(hipStreamSynchronize(stream));
-  } else if (impl != RKX_IMPL_D) {
+  } else if (sweep) {
     if (int rc = mem.alloc(reinterpret_cast<void **>(&inst_dev), sizeof(sweep_inst)))
       return rc;
     const sweep_inst hi = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
@@ -555,6 +575,9 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   auto enqueue_step = [&]() -> hipError_t {
     if (impl == RKX_IMPL_D)
       return launch_d_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], w, dy, err, stream);
+    if (gather)
+      return launch_gather_step(impl, dp, cf_dev, s, nz, st_dev, ybuf[0], ybuf[1], v, w, w1,
+                                stream);
     if (chain)
       return launch_chain(pattern, chain_dev, nullptr, ntiles, stream);
     return launch_sweep(pattern, cfg.waves, !sweep_single_reflect(pattern, s, dp.N), inst_dev,
@@ -609,7 +632,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     // Chunks of steps captured once as a graph.  The host keeps two chunks in
     // flight: it enqueues chunk k+1 (and a snapshot copy of the state) before
     // waiting on the snapshot taken after chunk k.
-    const int chunk = (impl == RKX_IMPL_D) ? 16 : 32;
+    const int chunk = fused ? 32 : 16;
     graph_holder gh;
     RKX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     hipError_t cap_err = hipSuccess;

@@ -150,6 +150,19 @@ hipError_t launch_eval_rng(const dev_ode &p, int i, int j, const double *y, doub
 hipError_t launch_d_step(const dev_ode &p, const dev_coef *cf, int s, state *st,
                          double *y0, double *y1, double *w, double *dy, double *err,
                          hipStream_t stream);
+// Gather forms A, E, F (src/impl/seq/A.c, E.c, F.c): non-zero patterns of the
+// raw table (zero_pattern, src/methods.c:24-39) as bit masks, bit i of rowA[l]
+// for A[l][i], of b for b[i], of e for b[i] - b_hat[i]
+struct gather_masks {
+  unsigned rowA[RKX_MAX_STAGES];
+  unsigned b, e;
+};
+gather_masks make_gather_masks(const rkx_method &m);
+// one attempted step of form RKX_IMPL_A/E/F: v holds s vectors of n, w0 (and
+// w1 for F) one each
+hipError_t launch_gather_step(int form, const dev_ode &p, const dev_coef *cf, int s,
+                              const gather_masks &nz, state *st, double *y0, double *y1,
+                              double *v, double *w0, double *w1, hipStream_t stream);
 
 struct sweep_cfg {
   int waves;     // wavefronts per workgroup (strip width 64*waves lanes)

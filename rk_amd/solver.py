@@ -9,11 +9,12 @@ import os
 
 import numpy as np
 
-from ._lib import (RKX_CTRL_DEVICE, RKX_CTRL_HOST, RKX_IMPL_D, RKX_IMPL_PIPED, RKX_IMPL_PIPEDLS,
-                   Opts, Stats, check, lib)
+from ._lib import (RKX_CTRL_DEVICE, RKX_CTRL_HOST, RKX_IMPL_A, RKX_IMPL_D, RKX_IMPL_E, RKX_IMPL_F,
+                   RKX_IMPL_PIPED, RKX_IMPL_PIPEDLS, Opts, Stats, check, lib)
 from .methods import method as _method
 
-IMPLS = {"D": RKX_IMPL_D, "PipeD": RKX_IMPL_PIPED, "PipeDls": RKX_IMPL_PIPEDLS}
+IMPLS = {"D": RKX_IMPL_D, "PipeD": RKX_IMPL_PIPED, "PipeDls": RKX_IMPL_PIPEDLS,
+         "A": RKX_IMPL_A, "E": RKX_IMPL_E, "F": RKX_IMPL_F}
 CTRLS = {"device": RKX_CTRL_DEVICE, "host": RKX_CTRL_HOST}
 
 

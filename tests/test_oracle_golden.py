@@ -27,8 +27,11 @@ def parse_dump(path):
 
 
 @pytest.mark.parametrize("name", sorted(INDEX))
-@pytest.mark.parametrize("impl", ["D", "PipeD"])
+@pytest.mark.parametrize("impl", ["D", "PipeD", "A", "E", "F"])
 def test_dump_trajectory_bitwise(orc, tmp_path, name, impl):
+    """Every attempted step of the oracle's schedule equals the reference's
+    DUMP trajectory; the reference's own A, E and F produced the same
+    trajectory on the cases marked "same_in"."""
     meta, prob, grid, d, z = load_case(name)
     p = orc.make_ode(prob, grid)
     y0 = orc.ode_start(p)
