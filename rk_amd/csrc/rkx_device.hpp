@@ -22,6 +22,7 @@ static __constant__ pow_tables dev_pow_tables =
 // wait for the global loads in flight.  Kernels that prefetch through such
 // pointers view them in the global address space instead.
 using gconst = const __attribute__((address_space(1))) double;
+using gbyte = const __attribute__((address_space(1))) char;
 using gmut = __attribute__((address_space(1))) double;
 __device__ __forceinline__ gconst *as_global(const double *p) { return (gconst *) p; }
 __device__ __forceinline__ gmut *as_global(double *p) { return (gmut *) p; }

@@ -90,7 +90,9 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   state *st = I.st;
   if (st->done)
     return;                                      // uniform over the instance's tiles
-  const int cur = st->cur;
+  // uniform, said explicitly: y, y_new and every row address below stay in
+  // scalar registers
+  const int cur = __builtin_amdgcn_readfirstlane(st->cur);
   gconst *__restrict__ y = as_global(cur ? I.y1 : I.y0);
   gmut *__restrict__ yn = as_global(cur ? I.y0 : I.y1);
   const int N = I.N;
@@ -135,7 +137,14 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   // y_old must have been loaded by then, i.e. in an iteration >= it0
   static_assert(DY <= 2 * (s - 1) && DY + 1 <= NS, "y_old prefetch outruns the band");
   static_assert(PF + 3 <= NS, "y ring too small for the prefetch");
-  gconst *yc = y + 2 * ccol + var;
+  // byte offset of this lane's component within a row (a row is addressed by
+  // a scalar base, the lane by a 32-bit vector offset)
+  const unsigned int lane_b = (2u * static_cast<unsigned int>(ccol) + var) * 8u;
+  auto at_row = [&](int kr) -> double {
+    const gbyte *row = reinterpret_cast<const gbyte *>(
+        y + static_cast<unsigned>(kr) * static_cast<unsigned>(N2));
+    return *reinterpret_cast<gconst *>(row + lane_b);
+  };
   // Ghost rows: a single reflection about row 0 / N-1, min(|k|, 2(N-1) - |k|),
   // covers every row a band loads once the grid is taller than the halo plus
   // the prefetch (plus one trip of the ring with full trips, which may run
@@ -144,7 +153,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   auto yrow = [&](int k) -> double {
     const int a = k < 0 ? -k : k;
     const int kr = GEN ? reflect(k, N) : min(a, 2 * (N - 1) - a);
-    return yc[static_cast<unsigned>(kr) * static_cast<unsigned>(N2)];
+    return at_row(kr);
   };
   static_for<PF + 2>([&](auto dt) {
     constexpr int d = decltype(dt)::value - 1;   // rows it0-1 .. it0+PF
@@ -202,7 +211,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
         // ahead, and y_old of the row the last stage finishes DY iterations
         // ahead (clamped into the band: rows outside it are never stored).
         const int kf = min(max(rr + DY - (s - 1), r0), r1 - 1);
-        yo[(U + DY) % NS] = yc[static_cast<unsigned>(kf) * static_cast<unsigned>(N2)];
+        yo[(U + DY) % NS] = at_row(kf);
         yr[(U + 1 + PF) % NS] = yrow(rr + 1 + PF);
         const int kl = rr - (s - 1);                 // row finishing this iteration
         const double yold = yo[U % NS];
@@ -243,8 +252,10 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
             vs = acc[ss][i];
           }
           const double pt = nP[i];
-          // Laplacian numerator ((vN + vS) + X) + m*Y - 4c, then the reaction
-          const double lap = __builtin_fma(-4.0, c, ((vn + vs) + nX[i]) + mY * nY[i]);
+          // Laplacian numerator ((vN + vS) + X) + m*Y - 4c, then the reaction.
+          // m*Y and 4c are exact (m is 1 or 2), so each multiply-add rounds
+          // once, as the separate add does.
+          const double lap = __builtin_fma(-4.0, c, __builtin_fma(mY, nY[i], (vn + vs) + nX[i]));
           double f;
           if constexpr (VAR == 0) {
             const double uuv = (c * c) * pt;
@@ -290,8 +301,10 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
                              2 * col + VAR, out, ynw);
             const double dv = fmax(fabs(ynw), fabs(yold));
             const bool dz = (dv == 0.0);
+            // lanes not stored and zero divisors give 0/dv: 0, or NaN for
+            // dv = 0 or NaN, which fmax skips
             const bool take = out && !dz;
-            double num = take ? fabs(acc[sl][s]) : 0.0, den = take ? dv : 1.0;
+            double num = take ? fabs(acc[sl][s]) : 0.0, den = dv;
             // opaque operands: otherwise the compiler folds 0/1 and puts the
             // division back under a branch
             __asm__("" : "+v"(num), "+v"(den));
