@@ -51,8 +51,16 @@ def _opts(impl="PipeDls", ctrl="device", h0=0.0, step_limit=0, tile_rows=0, tile
 
 def solve(problem, meth, t0, te, y0_dev, y_dev, tol, **kw):
     """Integrate on device buffers.  Returns (Stats, trace or None); the trace
-    holds (t, h, err) of every attempted step up to trace_cap."""
+    holds (t, h, err) of every attempted step up to trace_cap.
+
+    Ordering: without `stream` the solve runs on the library's own stream,
+    which is not ordered after torch's; when y0_dev is a tensor, torch's
+    current stream is synchronized first, so pending work that writes y0_dev
+    has finished.  The call returns after y_dev holds the result."""
     m = _method(meth) if isinstance(meth, str) else meth
+    if not kw.get("stream") and hasattr(y0_dev, "data_ptr"):
+        import torch
+        torch.cuda.current_stream(y0_dev.device).synchronize()
     o, trace = _opts(**kw)
     st = Stats()
     check(lib().rkx_solve(C.byref(problem.desc), C.byref(m), t0, te, C.c_void_p(_ptr(y0_dev)),
