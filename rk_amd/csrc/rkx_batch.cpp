@@ -111,37 +111,58 @@ sweep_cfg batch_cfg(const dev_ode &p, const rkx_method &m, const rkx_batch_opts 
   return c;
 }
 
-// Band height as the batch drains: the admitted shape keeps halo rows a small
-// share while many instances fill the CUs; once fewer tiles than two per CU
-// remain, the remaining fused instances get bands of half, a quarter, ...
-// (down to 16 rows) so that a lone large instance still spreads over the
-// chip instead of one workgroup sweeping all its rows.  The tiling never
-// changes an instance's arithmetic, only how its rows are shared out.
+// Band height as the batch drains.  The admitted shape keeps halo rows a
+// small share while many instances fill the CUs; as instances finish, the
+// remaining fused instances get bands of a half, a quarter, ... of it (down
+// to 8 rows), so that a few large instances still spread over the chip
+// instead of a few workgroups sweeping all their rows.  The divisor is the
+// one with the shortest modelled batched step: per (table, strip width)
+// launch, the row iterations of all its tiles shared over the resident
+// workgroup slots, but never less than its longest tile (bands recompute
+// 2(s-1) halo rows, so shorter bands cost more work in total and take less
+// time each).  The tiling never changes an instance's arithmetic, only how
+// its rows are shared out.
 int adapt_bands(rkx_batch *b)
 {
-  auto tiles_at = [&](int div) {
-    long long t = 0;
+  auto rows_at = [](const batch_inst &I, int div) {
+    return std::max(8, I.base.rows / div);   // bands of at least 8 rows
+  };
+  auto step_time = [&](int div) {
+    std::map<std::pair<int, int>, std::pair<double, double>> g;   // work, longest tile
     for (int i = 0; i < b->hi_slot; ++i) {
       const batch_inst &I = b->inst[static_cast<size_t>(i)];
-      if (!I.used || I.done || I.storage != RKX_STORAGE_LOW)
+      if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K == 0)
         continue;
-      if (I.K == 0) {
-        t += make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg;
-      } else {
-        const sweep_cfg c{I.base.waves, std::max(16, I.base.rows / div)};
-        t += make_sweep_inst(I.dp, I.m, c, nullptr, nullptr, nullptr, nullptr).ntiles;
-      }
+      const sweep_cfg c{I.base.waves, rows_at(I, div)};
+      const sweep_inst si = make_sweep_inst(I.dp, I.m, c, nullptr, nullptr, nullptr, nullptr);
+      const int band = sweep_band_rows(I.pattern, I.m.s, std::min(si.R, I.dp.N));
+      const double iters = band + 2.0 * (I.m.s - 1);
+      auto &e = g[{I.pattern, I.K}];
+      e.first += static_cast<double>(si.ntiles) * iters;
+      e.second = std::max(e.second, iters);
+    }
+    double t = 0.0;
+    for (const auto &kv : g) {
+      const int slots = b->cus * std::max(1, sweep_resident_blocks(kv.first.first,
+                                                                   kv.first.second));
+      t += std::max(kv.second.first / slots, kv.second.second);
     }
     return t;
   };
   int div = 1;
-  while (div < 64 && tiles_at(div) < 2LL * b->cus && tiles_at(2 * div) > tiles_at(div))
-    div *= 2;
+  double best = step_time(1);
+  for (int d = 2; d <= 64; d *= 2) {
+    const double t = step_time(d);
+    if (t < best) {
+      best = t;
+      div = d;
+    }
+  }
   for (int i = 0; i < b->hi_slot; ++i) {
     batch_inst &I = b->inst[static_cast<size_t>(i)];
     if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K == 0)
       continue;
-    const sweep_cfg c{I.base.waves, std::max(16, I.base.rows / div)};
+    const sweep_cfg c{I.base.waves, rows_at(I, div)};
     if (c.rows == I.cfg.rows)
       continue;
     I.cfg = c;
@@ -150,6 +171,36 @@ int adapt_bands(rkx_batch *b)
     RKX_HIP(hipMemcpyAsync(b->si_dev + i, &si, sizeof(si), hipMemcpyHostToDevice, b->stream));
   }
   return RKX_OK;
+}
+
+// Row iterations of one tile of a fused instance (its band, or the shorter
+// last band, plus the 2(s-1) halo rows).
+int tile_iters(const batch_inst &I, int t)
+{
+  const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
+  const int band = t % si.nbands;
+  const int rows = std::min(si.R, I.dp.N - band * si.R);
+  return sweep_band_rows(I.pattern, I.m.s, rows) + 2 * (I.m.s - 1);
+}
+
+// Longest tiles first, so that a batched step ends on short tiles instead of
+// leaving CUs idle behind a long one.  The sweep kernel gives each of the 8
+// XCDs a contiguous run of the table (workgroup b -> run b % 8, in order), so
+// the sorted tiles are dealt round-robin over the runs: every XCD works
+// through a mix, longest first.
+void order_tiles(const rkx_batch *b, std::vector<int2> &tiles)
+{
+  std::vector<std::pair<int, int2>> keyed;
+  keyed.reserve(tiles.size());
+  for (const int2 &t : tiles)
+    keyed.push_back({tile_iters(b->inst[static_cast<size_t>(t.x)], t.y), t});
+  std::stable_sort(keyed.begin(), keyed.end(),
+                   [](const auto &a, const auto &c) { return a.first > c.first; });
+  const size_t T = tiles.size(), q = T / 8, rm = T % 8;
+  for (size_t j = 0; j < T; ++j) {
+    const size_t x = j % 8, pos = j / 8;
+    tiles[x * q + std::min(x, rm) + pos] = keyed[j].second;
+  }
 }
 
 int rebuild_tables(rkx_batch *b)
@@ -177,6 +228,8 @@ int rebuild_tables(rkx_batch *b)
   }
   for (auto &kv : b->groups) {
     tile_group &g = kv.second;
+    if (g.K != 0)
+      order_tiles(b, g.tiles);
     if (g.tiles.size() > g.dev_cap) {
       if (g.dev)
         (void) hipFree(g.dev);
