@@ -207,6 +207,28 @@ def c2_matrix_leg(window=(300, 1300), impls=("D", "A", "E", "F", "PipeDls")):
     return rows
 
 
+def loop_order_leg():
+    """The step-form autotuner (rk_amd.autotune.solve_form) on three problems:
+    BRUSS2D N=2000 (fused sweep), an LV chain of 5e5 cells (fused chain
+    kernel) and 4e6 independent LV pairs (no fused kernel: D, A, E, F only;
+    "PipeDls" runs D there); DOPRI54, per-step times over attempted steps
+    20..120 and the form it picks."""
+    import torch
+    import rk_amd
+    from rk_amd.autotune import solve_form
+    cases = [("BRUSS2D N=2000", rk_amd.Bruss2D(N_GRID, 2e-3), 1e-6, 1.0, 1e-3),
+             ("LV chain M=5e5", rk_amd.LVChain(500_000, diff=1e-3), 1e-10, 40.0, 1e-4),
+             ("LV pairs 4e6", rk_amd.LV(4_000_000), 1e-10, 40.0, 1e-4)]
+    rows = []
+    for name, p, tol, te, h0 in cases:
+        y0 = torch.from_numpy(p.ode_start()).cuda()
+        best, times = solve_form(p, "DOPRI54", y0, tol, te=te, h0=h0)
+        rows.append({"problem": name, "best": best,
+                     "us_per_step": {k: v * 1e6 for k, v in times.items()}})
+        del y0
+    return rows
+
+
 def batched_leg(sample_every=1):
     """Mixed batch of 512 BRUSS2D instances (N uniform on 32..512 from seed 0,
     alpha log-uniform on [1e-3, 5e-3], noisy u0; DOPRI54 tol 1e-6, t in
@@ -361,7 +383,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--step-limit", type=int, default=0,
                     help="cap each solve at this many attempted steps (profiling runs)")
-    ap.add_argument("--leg", choices=("c2_matrix", "batched", "ensemble", "serving"),
+    ap.add_argument("--leg", choices=("c2_matrix", "loop_order", "batched", "ensemble", "serving"),
                     help="run only this leg and print its JSON (for profiling runs)")
     ap.add_argument("--grid", type=int, default=N_GRID,
                     help="BRUSS2D grid size of the main leg (profiling runs; the metric is "
@@ -377,7 +399,8 @@ def main():
         return 2
 
     if a.leg:
-        legs = {"c2_matrix": c2_matrix_leg, "batched": batched_leg, "ensemble": ensemble_leg,
+        legs = {"c2_matrix": c2_matrix_leg, "loop_order": loop_order_leg,
+                "batched": batched_leg, "ensemble": ensemble_leg,
                 "serving": lambda: serving_leg(a.serve_rate, a.serve_seconds)}
         print(json.dumps({"leg": a.leg, "result": legs[a.leg]()}))
         return 0
@@ -465,6 +488,7 @@ def main():
     if not a.no_batch and a.step_limit == 0:
         res["c2_matrix"] = c2_matrix_leg()
         print("c2 matrix done", file=sys.stderr, flush=True)
+        res["loop_order"] = loop_order_leg()
         res["batched"] = batched_leg()
         print("batched: %.2f s vs plain %.2f s" % (res["batched"]["batched_s"],
                                                    res["batched"]["plain_s"]),

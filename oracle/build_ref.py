@@ -113,6 +113,13 @@ REF_DROPIN = {
         dict(LV_COUNT=10000, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54")),
     "refmain_lv_verner65_t20": ("lv", "lv",
         dict(LV_COUNT=10000, T_END="20.0", TOL="1E-8", STEP_LIMIT=0, METHOD="VERNER65")),
+    # the same driver with this build's gather loop orders behind solver()
+    "refmain_bruss_c1_F": ("bruss2d-mix", "bruss2d",
+        dict(BRUSS_GRID_SIZE=128, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54",
+             RKX_DROPIN_IMPL="RKX_IMPL_F")),
+    "refmain_lv_dopri54_A": ("lv", "lv",
+        dict(LV_COUNT=10000, T_END="4.0", TOL="1E-6", H_INIT="1E-3", STEP_LIMIT=0, METHOD="DOPRI54",
+             RKX_DROPIN_IMPL="RKX_IMPL_A")),
 }
 REPO = os.path.dirname(HERE)
 DROPIN_DIR = os.path.join(REPO, "rk_amd", "csrc", "dropin")

@@ -65,7 +65,9 @@ REF_DRIVER = [("refmain_bruss_c1", "c1_bruss128_dopri54_tol1e-6", "seq_bruss_D_n
               ("refmain_bruss_verner65_tol1e-8", "bruss128_verner65_tol1e-8",
                "seq_bruss_D_n128_verner65_tol1e-8"),
               ("refmain_lv_dopri54", "lv10000_dopri54", "seq_lv_D_10000_dopri54"),
-              ("refmain_lv_verner65_t20", "lv10000_verner65_t20", "seq_lv_D_10000_verner65_t20")]
+              ("refmain_lv_verner65_t20", "lv10000_verner65_t20", "seq_lv_D_10000_verner65_t20"),
+              ("refmain_bruss_c1_F", "c1_bruss128_dopri54_tol1e-6", "seq_bruss_D_n128_dopri54_tol1e-6"),
+              ("refmain_lv_dopri54_A", "lv10000_dopri54", "seq_lv_D_10000_dopri54")]
 REF_BIN = os.path.join(os.path.dirname(HERE), "oracle", "_ref")
 # lines that name the solver or carry a measured time
 VARIES = re.compile(r"^(Solver type|Implementation variant|Kernel time|Total time):")
