@@ -102,7 +102,9 @@ enum {
                             keeps every stage vector of src/impl/seq/PipeD.c in registers,
                             so it has no full-storage variant (two vectors of n) */
   RKX_IMPL_PIPEDLS = 2,  /* stage-fused sweep, low storage: two vectors of n
-                            (src/impl/seq/PipeDls.c) */
+                            (src/impl/seq/PipeDls.c); without a fused kernel for the
+                            problem and table, the full-storage loop order below with
+                            the fewest vector passes per step */
   RKX_IMPL_A = 3,        /* gather form skipping zero coefficients: w_l = y + sum hA v_i,
                             then v_l = f(w_l) (src/impl/seq/A.c); s + 3 vectors of n */
   RKX_IMPL_E = 4,        /* the gather form without the zero skip (src/impl/seq/E.c) */

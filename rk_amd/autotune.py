@@ -58,8 +58,8 @@ FORMS = ("PipeDls", "D", "A", "E", "F")
 
 def solve_form(problem, meth, y0, tol, t0=0.0, te=1.0, h0=1e-3, window=(20, 120), forms=FORMS):
     """Time the step forms on one problem: the fused low-storage sweep
-    (PipeDls; a problem without a fused kernel runs the D stage kernels under
-    that name) and the full-storage loop orders D, A, E and F of the
+    (PipeDls; a problem without a fused kernel runs the full-storage loop
+    order with the least traffic under that name) and the full-storage loop orders D, A, E and F of the
     reference (src/impl/seq/D.c, A.c, E.c, F.c).  All of them give the same
     steps and results bitwise, so per-step times compare directly; each is
     the difference of two solves capped at window[0] and window[1] attempted

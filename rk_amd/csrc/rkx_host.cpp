@@ -414,6 +414,41 @@ int rkx_eval_comp_host(const rkx_ode *p, int i, double t, const double *y, doubl
   return RKX_OK;
 }
 
+// The full-storage loop order that moves the fewest vectors of n per step
+// for a table: D (block-inline.h:253-320) scatters into s+1 accumulators, A
+// and E gather each stage from the stage derivatives (A skipping the zero
+// entries of the table), F fuses each RHS with the next stage's gather.  All
+// give the same results bitwise.  Counts: reads + writes of n doubles.
+static int least_traffic_form(const rkx_method &m)
+{
+  const int s = m.s;
+  int d = (s + 2) + 5;                                   // first and last stage
+  for (int i = 1; i < s - 1; ++i)
+    d += 5 + 2 * (s - 1 - i);
+  int a = 2 + 1, e = 2 + 1;                              // stage 0; output reads y
+  for (int l = 1; l < s; ++l) {
+    int nz = 0;
+    for (int i = 0; i < l; ++i)
+      nz += (m.A[l][i] != 0.0);
+    a += 4 + nz;
+    e += 4 + l;
+  }
+  for (int i = 0; i < s; ++i) {
+    a += (i == 0 || m.b[i] != 0.0 || m.b[i] - m.b_hat[i] != 0.0) ? 1 : 0;
+    e += 1;
+  }
+  a += 1;                                                // y_new
+  e += 1;
+  int f = 3 + 2 + (s + 2);                               // stage 0, last stage, output
+  for (int l = 1; l < s - 1; ++l)
+    f += l + 4;
+  int best = RKX_IMPL_D, cost = d;
+  if (f < cost) { best = RKX_IMPL_F; cost = f; }
+  if (a < cost) { best = RKX_IMPL_A; cost = a; }
+  if (e < cost) { best = RKX_IMPL_E; cost = e; }
+  return best;
+}
+
 size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
 {
   if (check_ode(p) || check_method(m))
@@ -423,11 +458,13 @@ size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
   const dev_ode dp = make_dev_ode(*p);
   const bool fused = sweep_supported(dp, *m) || chain_supported(dp, *m);
   const size_t s = static_cast<size_t>(m->s);
+  if ((impl == RKX_IMPL_PIPED || impl == RKX_IMPL_PIPEDLS) && !fused)
+    impl = least_traffic_form(*m);
   if (impl == RKX_IMPL_A || impl == RKX_IMPL_E)
     return (2 + s + 1) * vec + fixed;            // y x2, v_0..v_{s-1}, w
   if (impl == RKX_IMPL_F)
     return (2 + s + 2) * vec + fixed;            // y x2, v_0..v_{s-1}, w x2
-  if (impl == RKX_IMPL_D || !fused)
+  if (impl == RKX_IMPL_D)
     return (2 + (s - 1) + 2) * vec + fixed;      // y x2, w_1..w_{s-1}, dy, err
   return 2 * vec + fixed;                        // y x2
 }
@@ -465,15 +502,16 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   int impl = o.impl;
   // fused forms: the row sweep for BRUSS2D, the segment kernel for the LV
   // chain; anything else (LV pairs, tables without a compiled pattern) runs
-  // the stage kernels of D.  A, E and F run as asked on every problem.
-  const bool gather = impl == RKX_IMPL_A || impl == RKX_IMPL_E || impl == RKX_IMPL_F;
+  // the full-storage loop order with the least traffic.  D, A, E and F run as
+  // asked on every problem.
   bool fused = impl == RKX_IMPL_PIPED || impl == RKX_IMPL_PIPEDLS;
   const bool chain = fused && chain_supported(dp, *m);
   if (fused && !chain && !sweep_supported(dp, *m)) {
-    impl = RKX_IMPL_D;
+    impl = least_traffic_form(*m);
     fused = false;
   }
   const bool sweep = fused && !chain;
+  const bool gather = impl == RKX_IMPL_A || impl == RKX_IMPL_E || impl == RKX_IMPL_F;
   const size_t n = static_cast<size_t>(p->n);
   const size_t vec = n * sizeof(double);
 
