@@ -254,7 +254,7 @@ int rkx_pow_host(const double *x, const double *y, double *p, size_t n);
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory (tight serving loops, harnesses that forbid allocation): rkx_solve
  * for the fused low-storage forms (impl RKX_IMPL_PIPEDLS or RKX_IMPL_PIPED;
- * BRUSS2D and LV chains with a table that has a fused kernel), the same
+ * BRUSS2D, LV pairs and LV chains with a table that has a fused kernel), the same
  * arguments and bitwise the same results.  y_dev serves as one of the two y
  * buffers; the other and the step state live in a 32 MiB device array the
  * library holds on each device it runs on until the process ends.  Makes no

@@ -25,6 +25,7 @@ SOURCES = [
     "rkx_methods.cpp",
     "rkx_kernels_d.hip",
     "rkx_kernels_gather.hip",
+    "rkx_kernels_lvpairs.hip",
     "rkx_kernels_sweep.hip",
     "rkx_kernels_sweep_dopri87.hip",
     "rkx_kernels_lv.hip",

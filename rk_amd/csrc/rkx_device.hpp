@@ -169,7 +169,42 @@ __device__ __forceinline__ void err_term(double e, double yn, double yo, double 
 // st->err_bits / st->zero_div.  Resets the accumulators for the next step.
 // The new h uses libm_pow, the C library's pow restated bitwise, so the
 // device reproduces the reference's step sizes without a host round trip.
-__device__ inline void dev_control(state *st)
+// Every workgroup thread t of nt computes its share of the premultiplied
+// coefficients of the next step from the raw table: the same products h*A,
+// h*b, h*(b - b_hat) as premult, each written to both places it is read from.
+__device__ inline void premult_block(const dev_method &m, int s, double h, dev_coef &c, int t,
+                                     int nt)
+{
+  const int total = s * s + 2 * s;
+  for (int idx = t; idx < total; idx += nt) {
+    if (idx < s * s) {
+      const int i = idx / s, j = idx - (idx / s) * s;
+      if (j < i) {
+        const double v = h * m.A[i * RKX_MAX_STAGES + j];
+        c.hA[i * RKX_MAX_STAGES + j] = v;
+        c.col[j][i - j - 1] = v;
+      }
+    } else if (idx < s * s + s) {
+      const int i = idx - s * s;
+      const double v = h * m.b[i];
+      c.hb[i] = v;
+      c.col[i][s - i - 1] = v;
+    } else {
+      const int i = idx - s * s - s;
+      const double v = h * m.e[i];
+      c.he[i] = v;
+      c.col[i][s - i] = v;
+    }
+  }
+}
+
+// Runs in exactly one thread after every workgroup has folded its maximum into
+// st->err_bits / st->zero_div.  Resets the accumulators for the next step.
+// The new h uses libm_pow, the C library's pow restated bitwise, so the
+// device reproduces the reference's step sizes without a host round trip.
+// Returns whether the next step's coefficients need premultiplying (device
+// control), which the caller's workgroup does with premult_block.
+__device__ inline bool dev_control(state *st)
 {
   const unsigned long long bits = atomicOr(&st->err_bits, 0ull);
   const unsigned int zero = atomicOr(&st->zero_div, 0u);
@@ -179,7 +214,7 @@ __device__ inline void dev_control(state *st)
   st->blocks_done = 0u;
   st->err = zero ? fmax(1.7E308, q) : q;
   if (st->ctrl == RKX_CTRL_HOST)
-    return;
+    return false;
   const int attempt = st->acc + st->rej;
   if (st->trace && attempt < st->trace_cap) {
     st->trace[3 * attempt + 0] = st->t;
@@ -189,7 +224,7 @@ __device__ inline void dev_control(state *st)
   const bool need = ctrl_decide(*st);
   const double p = need ? libm_pow(st->pow_x, st->pow_y, dev_pow_tables) : 0.0;
   ctrl_finish(*st, need, p);
-  premult(*st->meth, st->s, st->h, *st->coef);
+  return true;
 }
 
 // ---- reductions -----------------------------------------------------------------
@@ -235,9 +270,18 @@ __device__ inline void block_finish(state *st, double qmax, unsigned int zero, d
     is_last = (prev == nblocks - 1u);
   }
   __syncthreads();
-  if (is_last && threadIdx.x == 0) {
-    __threadfence();
-    dev_control(st);
+  if (is_last) {                       // uniform over the workgroup
+    __shared__ int pm;
+    __shared__ double hn;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      pm = dev_control(st) ? 1 : 0;
+      hn = st->h;
+    }
+    __syncthreads();
+    if (pm)
+      premult_block(*st->meth, st->s, hn, *st->coef, static_cast<int>(threadIdx.x),
+                    static_cast<int>(blockDim.x));
   }
 }
 

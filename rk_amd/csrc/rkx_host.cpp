@@ -456,7 +456,8 @@ size_t rkx_solve_footprint(const rkx_ode *p, const rkx_method *m, int impl)
   const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
   const size_t fixed = sizeof(state) + sizeof(dev_method) + sizeof(dev_coef);
   const dev_ode dp = make_dev_ode(*p);
-  const bool fused = sweep_supported(dp, *m) || chain_supported(dp, *m);
+  const bool fused =
+      sweep_supported(dp, *m) || chain_supported(dp, *m) || pairs_supported(dp, *m);
   const size_t s = static_cast<size_t>(m->s);
   if ((impl == RKX_IMPL_PIPED || impl == RKX_IMPL_PIPEDLS) && !fused)
     impl = least_traffic_form(*m);
@@ -501,16 +502,17 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   const int s = m->s;
   int impl = o.impl;
   // fused forms: the row sweep for BRUSS2D, the segment kernel for the LV
-  // chain; anything else (LV pairs, tables without a compiled pattern) runs
-  // the full-storage loop order with the least traffic.  D, A, E and F run as
-  // asked on every problem.
+  // chain, the per-pair kernel for LV pairs; anything else (tables without a
+  // compiled pattern or stage count) runs the full-storage loop order with
+  // the least traffic.  D, A, E and F run as asked on every problem.
   bool fused = impl == RKX_IMPL_PIPED || impl == RKX_IMPL_PIPEDLS;
   const bool chain = fused && chain_supported(dp, *m);
-  if (fused && !chain && !sweep_supported(dp, *m)) {
+  const bool pairs = fused && !chain && pairs_supported(dp, *m);
+  if (fused && !chain && !pairs && !sweep_supported(dp, *m)) {
     impl = least_traffic_form(*m);
     fused = false;
   }
-  const bool sweep = fused && !chain;
+  const bool sweep = fused && !chain && !pairs;
   const bool gather = impl == RKX_IMPL_A || impl == RKX_IMPL_E || impl == RKX_IMPL_F;
   const size_t n = static_cast<size_t>(p->n);
   const size_t vec = n * sizeof(double);
@@ -616,6 +618,8 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
     if (gather)
       return launch_gather_step(impl, dp, cf_dev, s, nz, st_dev, ybuf[0], ybuf[1], v, w, w1,
                                 stream);
+    if (pairs)
+      return launch_lv_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], stream);
     if (chain)
       return launch_chain(pattern, chain_dev, nullptr, ntiles, stream);
     return launch_sweep(pattern, cfg.waves, !sweep_single_reflect(pattern, s, dp.N), inst_dev,

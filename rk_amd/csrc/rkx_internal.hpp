@@ -150,6 +150,12 @@ hipError_t launch_eval_rng(const dev_ode &p, int i, int j, const double *y, doub
 hipError_t launch_d_step(const dev_ode &p, const dev_coef *cf, int s, state *st,
                          double *y0, double *y1, double *w, double *dy, double *err,
                          hipStream_t stream);
+// Stage-fused step of independent LV pairs (rkx_kernels_lvpairs.hip): every
+// table with 2, 3, 4, 7, 8 or 13 stages
+bool pairs_supported(const dev_ode &p, const rkx_method &m);
+hipError_t launch_lv_step(const dev_ode &p, const dev_coef *cf, int s, state *st, double *y0,
+                          double *y1, hipStream_t stream);
+
 // Gather forms A, E, F (src/impl/seq/A.c, E.c, F.c): non-zero patterns of the
 // raw table (zero_pattern, src/methods.c:24-39) as bit masks, bit i of rowA[l]
 // for A[l][i], of b for b[i], of e for b[i] - b_hat[i]

@@ -92,7 +92,8 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
     return fail(RKX_EINVAL, "rkx_solve_noalloc: trace and dump output need rkx_solve");
   const dev_ode dp = make_dev_ode(*p);
   const bool chain = chain_supported(dp, *m);
-  if (!chain && !sweep_supported(dp, *m))
+  const bool pairs = !chain && pairs_supported(dp, *m);
+  if (!chain && !pairs && !sweep_supported(dp, *m))
     return fail(RKX_EINVAL, "rkx_solve_noalloc: no fused kernel for this problem and method "
                             "(the D form needs full storage: use rkx_solve)");
   const size_t n = static_cast<size_t>(p->n);
@@ -152,7 +153,7 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
   if (chain) {
     ci = make_chain_inst(dp, *m, st_dev, cf_dev, ybuf[0], ybuf[1]);
     ntiles = ci.nwg;
-  } else {
+  } else if (!pairs) {
     cfg = sweep_default_cfg(dp, *m);
     if (o.tile_rows > 0)
       cfg.rows = o.tile_rows;
@@ -173,11 +174,13 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
   RKX_HIP(hipMemcpyAsync(st_dev, &hs, sizeof(hs), hipMemcpyHostToDevice, stream));
   if (chain)
     RKX_HIP(hipMemcpyAsync(inst_dev, &ci, sizeof(ci), hipMemcpyHostToDevice, stream));
-  else
+  else if (!pairs)
     RKX_HIP(hipMemcpyAsync(inst_dev, &si, sizeof(si), hipMemcpyHostToDevice, stream));
   RKX_HIP(hipStreamSynchronize(stream));
 
   auto step = [&]() -> hipError_t {
+    if (pairs)
+      return launch_lv_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], stream);
     if (chain)
       return launch_chain(pattern, static_cast<const chain_inst *>(inst_dev), nullptr, ntiles,
                           stream);

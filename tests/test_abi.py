@@ -65,44 +65,27 @@ def test_lv_descriptors(orc):
     assert q.ode_size == 18 and q.ode_acc_dist() == 2
 
 
-def _passes(name, impl):
-    """Vector passes (reads + writes of n doubles) of one step of a loop order,
-    counted from the reference's kernels (block-inline.h:30-320)."""
-    import rk_amd
-    m = rk_amd.method(name)
-    s = m.s
-    if impl == "D":
-        return (s + 2) + sum(5 + 2 * (s - 1 - i) for i in range(1, s - 1)) + 5
-    if impl == "F":
-        return 3 + sum(l + 4 for l in range(1, s - 1)) + 2 + (s + 2)
-    skip = impl == "A"
-    nz = [sum(1 for i in range(l) if not skip or m.A[l][i] != 0.0) for l in range(s)]
-    out = 1 + sum(1 for i in range(1, s) if not skip or m.b[i] != 0.0 or m.b[i] != m.b_hat[i])
-    return 2 + sum(4 + nz[l] for l in range(1, s)) + out + 2
-
-
 @pytest.mark.parametrize("name", ["HEUN2", "RK4", "SSPRK3", "RKF23", "DOPRI54", "VERNER65",
                                   "DOPRI87"])
-def test_fallback_is_least_traffic_form(name):
-    """Without a fused kernel (LV pairs) the low-storage request runs the
-    full-storage loop order that moves the fewest vectors; its footprint
-    tells which one: D holds s + 3 vectors of n, A and E s + 3, F s + 4."""
+def test_footprints_per_form(name):
+    """Resident vectors of n per step form: the fused forms two (BRUSS2D, LV
+    pairs, LV chain), D, A and E s + 3, F s + 4.  The LV chain has no DOPRI87
+    kernel: there a low-storage request runs the loop order with the fewest
+    vector passes, A (120 against F 130, E 143, D 207 for DOPRI87's sparse
+    table), and holds A's vectors."""
     import ctypes as C
     import rk_amd
     from rk_amd._lib import lib
-    p = rk_amd.LV(1000)
     m = rk_amd.method(name)
-    vec = p.n * 8
-    fp = {impl: lib().rkx_solve_footprint(C.byref(p.desc), C.byref(m), code) // vec
-          for impl, code in (("D", 0), ("PipeDls", 2), ("A", 3), ("E", 4), ("F", 5))}
-    assert fp["D"] == fp["A"] == fp["E"] == m.s + 3 and fp["F"] == m.s + 4
-    passes = {impl: _passes(name, impl) for impl in ("D", "F", "A", "E")}
-    best = min(passes, key=lambda k: (passes[k], ["D", "F", "A", "E"].index(k)))
-    assert fp["PipeDls"] == fp[best]
-    if name in ("DOPRI54", "VERNER65"):
-        assert best == "F"
-    if name == "DOPRI87":
-        assert best == "A"
+    for p in (rk_amd.Bruss2D(40), rk_amd.LV(1000), rk_amd.LVChain(1000)):
+        vec = p.n * 8
+        fp = {impl: lib().rkx_solve_footprint(C.byref(p.desc), C.byref(m), code) // vec
+              for impl, code in (("D", 0), ("PipeDls", 2), ("A", 3), ("E", 4), ("F", 5))}
+        assert fp["D"] == fp["A"] == fp["E"] == m.s + 3 and fp["F"] == m.s + 4
+        if isinstance(p, rk_amd.LVChain) and name == "DOPRI87":
+            assert fp["PipeDls"] == m.s + 3
+        else:
+            assert fp["PipeDls"] == 2
 
 
 def test_bad_arguments_raise():

@@ -239,6 +239,48 @@ def test_chain_host_control_bitwise(rk, orc, M, meth, impl):
     assert np.array_equal(y, yo)
 
 
+@pytest.mark.parametrize("meth", ["HEUN2", "SSPRK3", "RKF23", "RK4", "DOPRI54", "VERNER65",
+                                  "DOPRI87"])
+def test_lv_pairs_fused_bitwise(rk, orc, meth):
+    """LV pairs through the per-pair fused kernel (low storage): every attempted
+    step (t, h, err) and the final vector equal the oracle's seq D, with
+    constants where d0 != c1 and start values off the reference's."""
+    consts = (1.3, 0.7, 0.45, 0.9)
+    p = rk.LV(4099, *consts)
+    po = orc.make_ode("lv", 4099, lv=consts)
+    y0 = 0.5 + np.random.default_rng(3).random(p.n)
+    m = rk.method(meth)
+    tol = 1e-9 if m.has_emb else 0.0
+    te = 6.0 if m.has_emb else 0.3
+    yo, so, tro = orc.solve(po, meth, 0.0, te, y0, tol, h0=1e-3, trace_cap=20000)
+    for ctrl in ("host", "device"):
+        print("lv pairs %s %s" % (meth, ctrl), flush=True)
+        y, st, tr = run_gpu(rk, p, meth, 0.0, te, y0, tol, h0=1e-3, impl="PipeDls", ctrl=ctrl,
+                            trace_cap=20000)
+        assert (st.steps_acc, st.steps_rej) == (so["acc"], so["rej"])
+        assert np.array_equal(tr, tro)
+        assert np.array_equal(y, yo)
+        assert st.resident_bytes < 3 * p.n * 8       # two vectors: the low-storage form
+
+
+def test_lv_pairs_fused_equals_d_form_large(rk):
+    """4e6 pairs (n = 8e6, as at C2), a few steps: fused ≡ D bitwise."""
+    import torch
+    p = rk.LV(4_000_000)
+    y0 = dev(0.5 + np.random.default_rng(5).random(p.n))
+    out = {}
+    for impl in ("PipeDls", "D"):
+        print("lv pairs 4e6 %s" % impl, flush=True)
+        y = torch.empty_like(y0)
+        st, _ = rk.solve(p, "VERNER65", 0.0, 10.0, y0, y, 1e-10, h0=1e-4, step_limit=25,
+                         impl=impl)
+        torch.cuda.synchronize()
+        out[impl] = (y.cpu().numpy(), (st.steps_acc, st.steps_rej))
+        del y
+    assert out["PipeDls"][1] == out["D"][1]
+    assert np.array_equal(out["PipeDls"][0], out["D"][0])
+
+
 def test_chain_device_control_bitwise(rk, orc):
     M = 5000
     p = rk.LVChain(M)

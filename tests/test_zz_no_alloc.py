@@ -1,7 +1,7 @@
 """rkx_solve_noalloc (the solve that allocates nothing) against rkx_solve
 through the C ABI: identical step counts and bitwise the same y, on two
 calls in a row whose second is smaller (the scratch array keeps the first
-call's contents), for BRUSS2D grids and LV chains, with device and host step
+call's contents), for BRUSS2D grids, LV chains and LV pairs, with device and host step
 control, in place (y0 == y) and into a separate output."""
 import ctypes as C
 
@@ -43,7 +43,9 @@ def test_noalloc_equals_solve_two_calls(rk, ctrl):
     cases = [(rk.Bruss2D(300, 2.5e-3), "DOPRI54", 0.1, 1e-6, False),
              (rk.Bruss2D(90, 2e-3), "VERNER65", 0.3, 1e-8, True),
              (rk.LVChain(60_000, diff=1e-3), "DOPRI54", 1.0, 1e-6, False),
-             (rk.LVChain(7_001, diff=2e-3), "RKF23", 1.0, 1e-5, True)]
+             (rk.LVChain(7_001, diff=2e-3), "RKF23", 1.0, 1e-5, True),
+             (rk.LV(50_000), "DOPRI54", 4.0, 1e-8, False),
+             (rk.LV(3_001), "VERNER65", 2.0, 1e-8, True)]
     for p, meth, te, tol, inplace in cases:
         y0 = p.ode_start_seeded(7, 1, 0.01)
         ya, sa = call("rkx_solve_noalloc", p, meth, te, y0, tol, 1e-3, inplace=inplace, ctrl=ctrl)
@@ -65,7 +67,7 @@ def test_noalloc_limits(rk):
     with pytest.raises(RkxError) as e:
         call("rkx_solve_noalloc", p, "DOPRI54", 0.1, p.ode_start(), 1e-6, 1e-3, impl="D")
     assert e.value.code == -1
-    lv = rk.LV(100)
+    ch = rk.LVChain(100)                      # the chain kernel has no DOPRI87 form
     with pytest.raises(RkxError) as e:
-        call("rkx_solve_noalloc", lv, "DOPRI54", 0.1, lv.ode_start(), 1e-6, 1e-3)
+        call("rkx_solve_noalloc", ch, "DOPRI87", 0.1, ch.ode_start(), 1e-6, 1e-3)
     assert e.value.code == -1
