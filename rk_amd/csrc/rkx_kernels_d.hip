@@ -18,10 +18,12 @@ namespace rkx {
 namespace {
 
 constexpr int kBlock = 256;
-// workgroups of the last stage (grid-stride): 8 per CU of MI355X, enough to
-// stream at full bandwidth, few enough that the per-workgroup atomics on the
-// step state do not serialize
-constexpr int kLastBlocks = 2048;
+// the last stage runs grid-stride in 256 workgroups of 16 waves (4 per SIMD,
+// enough to stream at full bandwidth): each workgroup folds its error maximum
+// into the step state with atomics on one word, which queue at the end of a
+// step, so few large workgroups finish sooner than many small ones
+constexpr int kLastThreads = 1024;
+constexpr int kLastBlocks = 256;
 
 __device__ __forceinline__ double *ycur(state *st, double *y0, double *y1)
 {
@@ -92,14 +94,15 @@ __global__ __launch_bounds__(kBlock) void k_d_interm(dev_ode p, const dev_coef *
   *reinterpret_cast<double2 *>(err + i) = e;
 }
 
-__global__ __launch_bounds__(kBlock) void k_d_last(dev_ode p, const dev_coef *__restrict__ m,
-                                                   int s, state *st, double *y0, double *y1,
-                                                   double *__restrict__ w,
-                                                   double *__restrict__ dy,
-                                                   double *__restrict__ err)
+__global__ __launch_bounds__(kLastThreads) void k_d_last(dev_ode p,
+                                                         const dev_coef *__restrict__ m, int s,
+                                                         state *st, double *y0, double *y1,
+                                                         double *__restrict__ w,
+                                                         double *__restrict__ dy,
+                                                         double *__restrict__ err)
 {
-  __shared__ double red[kBlock / 64];
-  __shared__ unsigned int zred[kBlock / 64];
+  __shared__ double red[kLastThreads / 64];
+  __shared__ unsigned int zred[kLastThreads / 64];
   if (st->done)
     return;                       // uniform: every block of the grid returns
   const int cur = st->cur;
@@ -109,8 +112,8 @@ __global__ __launch_bounds__(kBlock) void k_d_last(dev_ode p, const dev_coef *__
   unsigned int zero = 0u;
   // grid-stride: a bounded number of workgroups folds its maximum into the
   // step state (one atomic each), instead of one per 256 cells
-  for (int cell = blockIdx.x * kBlock + threadIdx.x; cell < p.cells;
-       cell += gridDim.x * kBlock) {
+  for (int cell = blockIdx.x * kLastThreads + threadIdx.x; cell < p.cells;
+       cell += gridDim.x * kLastThreads) {
     const size_t n = static_cast<size_t>(p.n);
     const int stage = s - 1;
     double F0, F1;
@@ -170,8 +173,9 @@ hipError_t launch_d_step(const dev_ode &p, const dev_coef *m, int s, state *st, 
   for (int i = 1; i < s - 1; ++i)
     hipLaunchKernelGGL(k_d_interm, dim3(grid), dim3(kBlock), 0, stream, p, m, s, i, st, w, dy,
                        err);
-  hipLaunchKernelGGL(k_d_last, dim3(std::min(grid, kLastBlocks)), dim3(kBlock), 0, stream, p, m,
-                     s, st, y0, y1, w, dy, err);
+  const int last_grid = std::min((p.cells + kLastThreads - 1) / kLastThreads, kLastBlocks);
+  hipLaunchKernelGGL(k_d_last, dim3(last_grid), dim3(kLastThreads), 0, stream, p, m, s, st, y0,
+                     y1, w, dy, err);
   return hipGetLastError();
 }
 

@@ -26,7 +26,10 @@ namespace rkx {
 namespace {
 
 constexpr int kGBlock = 256;
-constexpr int kGLastBlocks = 2048;   // bounded workgroups for the step-state atomics
+// the output kernel: 256 workgroups of 16 waves, grid-stride (few workgroups
+// fold their error maxima into the step state: those atomics queue)
+constexpr int kGOutThreads = 1024;
+constexpr int kGOutBlocks = 256;
 
 __device__ __forceinline__ const double *gcur(const state *st, const double *y0, const double *y1)
 {
@@ -120,13 +123,14 @@ __global__ __launch_bounds__(kGBlock) void k_g_rhs_gather(dev_ode p,
 // weight's table entry is non-zero, bits of nzb / nze), y_new = y + dy into the
 // other buffer, error max-norm, step control in the last workgroup
 template <bool SKIP>
-__global__ __launch_bounds__(kGBlock) void k_g_output(dev_ode p, const dev_coef *__restrict__ m,
-                                                      int s, unsigned nzb, unsigned nze,
-                                                      state *st, double *y0, double *y1,
-                                                      const double *__restrict__ v)
+__global__ __launch_bounds__(kGOutThreads) void k_g_output(dev_ode p,
+                                                           const dev_coef *__restrict__ m, int s,
+                                                           unsigned nzb, unsigned nze, state *st,
+                                                           double *y0, double *y1,
+                                                           const double *__restrict__ v)
 {
-  __shared__ double red[kGBlock / 64];
-  __shared__ unsigned int zred[kGBlock / 64];
+  __shared__ double red[kGOutThreads / 64];
+  __shared__ unsigned int zred[kGOutThreads / 64];
   if (st->done)
     return;
   const int cur = st->cur;
@@ -135,8 +139,8 @@ __global__ __launch_bounds__(kGBlock) void k_g_output(dev_ode p, const dev_coef 
   const size_t n = static_cast<size_t>(p.n);
   double qmax = 0.0;
   unsigned int zero = 0u;
-  for (int cell = blockIdx.x * kGBlock + threadIdx.x; cell < p.cells;
-       cell += gridDim.x * kGBlock) {
+  for (int cell = blockIdx.x * kGOutThreads + threadIdx.x; cell < p.cells;
+       cell += gridDim.x * kGOutThreads) {
     const int i = 2 * cell;
     const double2 V0 = *reinterpret_cast<const double2 *>(v + i);
     double e0 = m->he[0] * V0.x, e1 = m->he[0] * V0.y;
@@ -172,6 +176,8 @@ hipError_t launch_gather_step(int form, const dev_ode &p, const dev_coef *m, int
 {
   const int grid = (p.cells + kGBlock - 1) / kGBlock;
   const dim3 g(grid), b(kGBlock);
+  const dim3 go(std::min((p.cells + kGOutThreads - 1) / kGOutThreads, kGOutBlocks)),
+      bo(kGOutThreads);
   if (form == RKX_IMPL_F) {
     double *wc = w0, *wn = w1;
     for (int l = 0; l < s - 1; ++l) {
@@ -179,7 +185,7 @@ hipError_t launch_gather_step(int form, const dev_ode &p, const dev_coef *m, int
       std::swap(wc, wn);
     }
     hipLaunchKernelGGL(k_g_rhs, g, b, 0, stream, p, s - 1, st, y0, y1, wc, v);
-    hipLaunchKernelGGL((k_g_output<false>), dim3(std::min(grid, kGLastBlocks)), b, 0, stream, p,
+    hipLaunchKernelGGL((k_g_output<false>), go, bo, 0, stream, p,
                        m, s, nz.b, nz.e, st, y0, y1, v);
   } else {
     const bool skip = (form == RKX_IMPL_A);
@@ -194,10 +200,10 @@ hipError_t launch_gather_step(int form, const dev_ode &p, const dev_coef *m, int
       hipLaunchKernelGGL(k_g_rhs, g, b, 0, stream, p, l, st, y0, y1, w0, v);
     }
     if (skip)
-      hipLaunchKernelGGL((k_g_output<true>), dim3(std::min(grid, kGLastBlocks)), b, 0, stream, p,
+      hipLaunchKernelGGL((k_g_output<true>), go, bo, 0, stream, p,
                          m, s, nz.b, nz.e, st, y0, y1, v);
     else
-      hipLaunchKernelGGL((k_g_output<false>), dim3(std::min(grid, kGLastBlocks)), b, 0, stream,
+      hipLaunchKernelGGL((k_g_output<false>), go, bo, 0, stream,
                          p, m, s, nz.b, nz.e, st, y0, y1, v);
   }
   return hipGetLastError();
