@@ -6,7 +6,7 @@ sizes reach:
   drains, every instance bitwise equal to its plain solve, and the stiffest
   instances bitwise equal to the oracle;
 * LV chains of 10^4 and 6*10^5 cells: the second has more segments (595)
-  than an instance's workgroup cap (512), so workgroups loop over segments;
+  than an instance's workgroup cap (256), so workgroups loop over segments;
   fused kernel, D form and oracle bitwise equal;
 * BRUSS2D N=2000 VERNER65 at tol 1e-8 over the whole interval through the D
   form, against the reference's known answer.
