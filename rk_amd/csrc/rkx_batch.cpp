@@ -30,7 +30,8 @@ struct batch_inst {
   bool done = false;
   int storage = RKX_STORAGE_LOW;
   int pattern = 0;
-  int K = 0;                            // strip waves of the row sweep; 0: chain kernel
+  int K = 0;                            // strip waves of the row sweep; 0: chain kernel;
+                                        // -1: LV-pairs step kernel (its own launch)
   rkx_ode ode{};
   rkx_method m{};
   dev_ode dp{};
@@ -131,7 +132,7 @@ int adapt_bands(rkx_batch *b)
     std::map<std::pair<int, int>, std::pair<double, double>> g;   // work, longest tile
     for (int i = 0; i < b->hi_slot; ++i) {
       const batch_inst &I = b->inst[static_cast<size_t>(i)];
-      if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K == 0)
+      if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K <= 0)
         continue;
       const sweep_cfg c{I.base.waves, rows_at(I, div)};
       const sweep_inst si = make_sweep_inst(I.dp, I.m, c, nullptr, nullptr, nullptr, nullptr);
@@ -160,7 +161,7 @@ int adapt_bands(rkx_batch *b)
   }
   for (int i = 0; i < b->hi_slot; ++i) {
     batch_inst &I = b->inst[static_cast<size_t>(i)];
-    if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K == 0)
+    if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K <= 0)
       continue;
     const sweep_cfg c{I.base.waves, rows_at(I, div)};
     if (c.rows == I.cfg.rows)
@@ -213,7 +214,7 @@ int rebuild_tables(rkx_batch *b)
   }
   for (int i = 0; i < b->hi_slot; ++i) {
     const batch_inst &I = b->inst[static_cast<size_t>(i)];
-    if (!I.used || I.done || I.storage != RKX_STORAGE_LOW)
+    if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K < 0)
       continue;
     tile_group &g = b->groups[{I.pattern, I.K}];
     g.pattern = I.pattern;
@@ -265,10 +266,14 @@ int enqueue_batched_step(rkx_batch *b)
   }
   for (int i = 0; i < b->hi_slot; ++i) {
     batch_inst &I = b->inst[static_cast<size_t>(i)];
-    if (!I.used || I.done || I.storage != RKX_STORAGE_FULL)
+    if (!I.used || I.done)
       continue;
-    RKX_HIP(launch_d_step(I.dp, b->cf_dev + i, I.m.s, b->st_dev + i, I.y[0], I.y[1], I.w, I.dy,
-                          I.err, b->stream));
+    if (I.storage == RKX_STORAGE_LOW && I.K < 0)
+      RKX_HIP(launch_lv_step(I.dp, b->cf_dev + i, I.m.s, b->st_dev + i, I.y[0], I.y[1],
+                             b->stream));
+    else if (I.storage == RKX_STORAGE_FULL)
+      RKX_HIP(launch_d_step(I.dp, b->cf_dev + i, I.m.s, b->st_dev + i, I.y[0], I.y[1], I.w,
+                            I.dy, I.err, b->stream));
   }
   return RKX_OK;
 }
@@ -454,8 +459,9 @@ size_t rkx_batch_footprint(const rkx_ode *p, const rkx_method *m, int storage)
   const size_t vec = static_cast<size_t>(p->n) * sizeof(double);
   const dev_ode dp = make_dev_ode(*p);
   if (storage == RKX_STORAGE_AUTO)
-    storage = (sweep_supported(dp, *m) || chain_supported(dp, *m)) ? RKX_STORAGE_LOW
-                                                                     : RKX_STORAGE_FULL;
+    storage = (sweep_supported(dp, *m) || chain_supported(dp, *m) || pairs_supported(dp, *m))
+                  ? RKX_STORAGE_LOW
+                  : RKX_STORAGE_FULL;
   if (storage == RKX_STORAGE_LOW)
     return 2 * vec;
   return (2 + (m->s - 1) + 2) * vec;
@@ -472,7 +478,8 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
     return rc;
   const dev_ode dp = make_dev_ode(*p);
   const bool chain = chain_supported(dp, *m);
-  const bool fused = chain || sweep_supported(dp, *m);
+  const bool pairs = !chain && pairs_supported(dp, *m);
+  const bool fused = chain || pairs || sweep_supported(dp, *m);
   if (storage == RKX_STORAGE_AUTO)
     storage = fused ? RKX_STORAGE_LOW : RKX_STORAGE_FULL;
   if (storage == RKX_STORAGE_LOW && !fused)
@@ -538,7 +545,10 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
   premult(I.dm, m->s, hs.h, hc);
   I.snap = hs;
   I.done = hs.done;
-  if (storage == RKX_STORAGE_LOW && chain) {
+  if (storage == RKX_STORAGE_LOW && pairs) {
+    I.pattern = 0;
+    I.K = -1;
+  } else if (storage == RKX_STORAGE_LOW && chain) {
     I.pattern = sweep_pattern(*m);
     I.K = 0;
     const chain_inst ci =
@@ -665,7 +675,7 @@ int rkx_batch_tile(rkx_batch *b, int id, int *waves, int *rows)
   const batch_inst &I = b->inst[static_cast<size_t>(id)];
   *waves = 0;
   *rows = 0;
-  if (I.storage == RKX_STORAGE_LOW && I.K != 0) {
+  if (I.storage == RKX_STORAGE_LOW && I.K > 0) {
     *waves = I.cfg.waves;
     *rows = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).R;
   }

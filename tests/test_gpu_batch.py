@@ -127,7 +127,8 @@ def test_continuous_batching_and_budget(rk):
 
 def test_full_storage_and_mixed_kinds(rk):
     """Full-storage (D form) instances -- an LV chain, a VERNER65 grid, a
-    DOPRI87 grid without a fused kernel -- in the same batch as fused ones."""
+    DOPRI87 grid without a fused kernel, LV pairs -- in the same batch as fused
+    ones, among them LV pairs through their own per-pair kernel."""
     from rk_amd.workloads import bruss_instance, chain_instance
     cases = [(*bruss_instance(30, 2e-3, 0), "DOPRI54", "low"),
              (*bruss_instance(30, 2e-3, 0), "DOPRI54", "full"),
@@ -136,6 +137,10 @@ def test_full_storage_and_mixed_kinds(rk):
              (*chain_instance(2100, 1), "VERNER65", "low"),
              (*bruss_instance(20, 2e-3, 4), "VERNER65", "auto"),
              (*bruss_instance(12, 2e-3, 5), "DOPRI87", "auto")]
+    lv = rk.LV(5000, 1.3, 0.7, 0.45, 0.9)      # LV pairs: the per-pair fused kernel (low)
+    lv_y0 = 0.5 + np.random.default_rng(9).random(lv.n)
+    cases += [(lv, lv_y0, "DOPRI54", "auto"), (lv, lv_y0, "VERNER65", "full"),
+              (lv, lv_y0, "DOPRI87", "low")]
     with rk.Batch() as b:
         ids = [b.add(p, m, 0.0, 0.3, y0, 1e-7, h0=1e-3, storage=sto) for p, y0, m, sto in cases]
         assert b.run(100000) == 0
