@@ -178,7 +178,8 @@ def form_passes(meth, impl):
     return 2
 
 
-def c2_matrix_leg(window=(300, 1300), impls=("D", "A", "E", "F", "PipeDls")):
+def c2_matrix_leg(window=(300, 1300), impls=("D", "A", "E", "F", "PipeDls"),
+                  methods=("DOPRI54", "VERNER65", "DOPRI87"), tols=(1e-6, 1e-8)):
     """BRUSS2D N=2000, t in [0, 1], h0 1e-3: {DOPRI54, VERNER65, and the 13-stage
     DOPRI87} x tol {1e-6, 1e-8} x {full-storage loop orders D, A, E, F,
     low-storage fused PipeDls}; each timed over attempted steps
@@ -192,8 +193,8 @@ def c2_matrix_leg(window=(300, 1300), impls=("D", "A", "E", "F", "PipeDls")):
     y = torch.empty_like(y0)
     torch.cuda.synchronize()
     rows = []
-    for meth in ("DOPRI54", "VERNER65", "DOPRI87"):
-        for tol in (1e-6, 1e-8):
+    for meth in methods:
+        for tol in tols:
             for impl in impls:
                 kw = dict(h0=1e-3, impl=impl)
                 a, _ = rk_amd.solve(p, meth, 0.0, 1.0, y0, y, tol, step_limit=window[0], **kw)
@@ -388,6 +389,10 @@ def main():
     ap.add_argument("--grid", type=int, default=N_GRID,
                     help="BRUSS2D grid size of the main leg (profiling runs; the metric is "
                          "quoted at 2000)")
+    ap.add_argument("--matrix-methods", default="DOPRI54,VERNER65,DOPRI87",
+                    help="tables of the C2 matrix leg (profiling runs)")
+    ap.add_argument("--matrix-tols", default="1e-6,1e-8",
+                    help="tolerances of the C2 matrix leg (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run 5 plain device copies of n doubles, the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -399,7 +404,10 @@ def main():
         return 2
 
     if a.leg:
-        legs = {"c2_matrix": c2_matrix_leg, "loop_order": loop_order_leg,
+        mm = tuple(a.matrix_methods.split(","))
+        mt = tuple(float(x) for x in a.matrix_tols.split(","))
+        legs = {"c2_matrix": lambda: c2_matrix_leg(methods=mm, tols=mt),
+                "loop_order": loop_order_leg,
                 "batched": batched_leg, "ensemble": ensemble_leg,
                 "serving": lambda: serving_leg(a.serve_rate, a.serve_seconds)}
         print(json.dumps({"leg": a.leg, "result": legs[a.leg]()}))
