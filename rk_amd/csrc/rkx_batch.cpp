@@ -179,9 +179,7 @@ int adapt_bands(rkx_batch *b)
 int tile_iters(const batch_inst &I, int t)
 {
   const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
-  const int band = t % si.nbands;
-  const int rows = std::min(si.R, I.dp.N - band * si.R);
-  return sweep_band_rows(I.pattern, I.m.s, rows) + 2 * (I.m.s - 1);
+  return sweep_tile_iters(I.pattern, I.m.s, si, t);
 }
 
 // Longest tiles first, so that a batched step ends on short tiles instead of
@@ -219,11 +217,15 @@ int rebuild_tables(rkx_batch *b)
     tile_group &g = b->groups[{I.pattern, I.K}];
     g.pattern = I.pattern;
     g.K = I.K;
-    if (I.K != 0 && !sweep_single_reflect(I.pattern, I.m.s, I.dp.N))
-      g.gen = true;
-    const int nt =
-        I.K == 0 ? make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg
-                 : make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).ntiles;
+    int nt = 0;
+    if (I.K == 0) {
+      nt = make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg;
+    } else {
+      const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
+      nt = si.ntiles;
+      if (!sweep_inst_single_reflect(I.pattern, I.m.s, si))
+        g.gen = true;
+    }
     for (int t = 0; t < nt; ++t)
       g.tiles.push_back(make_int2(i, t));
   }

@@ -592,6 +592,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   sweep_inst *inst_dev = nullptr;
   chain_inst *chain_dev = nullptr;
   int ntiles = 0;
+  bool gen_rows = false;   // the general ghost-row form (sweep_inst_single_reflect)
   const int pattern = sweep_pattern(*m);
   if (sweep && sweep_resident_blocks(pattern, cfg.waves) < 1)
     return fail(RKX_EINVAL, "rkx_solve: tile_waves wider than this table's fused kernel "
@@ -608,6 +609,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
       return rc;
     const sweep_inst hi = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
     ntiles = hi.ntiles;
+    gen_rows = !sweep_inst_single_reflect(pattern, s, hi);
     RKX_HIP(hipMemcpyAsync(inst_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));
     RKX_HIP(hipStreamSynchronize(stream));
   }
@@ -622,8 +624,7 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
       return launch_lv_step(dp, cf_dev, s, st_dev, ybuf[0], ybuf[1], stream);
     if (chain)
       return launch_chain(pattern, chain_dev, nullptr, ntiles, stream);
-    return launch_sweep(pattern, cfg.waves, !sweep_single_reflect(pattern, s, dp.N), inst_dev,
-                        nullptr, ntiles, stream);
+    return launch_sweep(pattern, cfg.waves, gen_rows, inst_dev, nullptr, ntiles, stream);
   };
 
   state *hs_pinned = nullptr;   // [0], [1]: snapshots of two chunks in flight

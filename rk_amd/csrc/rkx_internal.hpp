@@ -177,12 +177,20 @@ struct sweep_cfg {
 // One BRUSS2D instance as the fused kernel sees it (device memory).  A launch
 // covers a table of (instance, tile) pairs, so one kernel steps many
 // instances, each with its own state, coefficients and step control.
+//
+// Columns: nwide strips of OC output columns, each band of a strip one
+// workgroup.  When the columns left over after the widest strips fit a strip
+// of half the lanes (split), they form a narrow last strip whose bands are
+// paired: one workgroup carries two bands of it, each on half of its waves,
+// so the narrow strip costs half a strip of workgroups.
 struct sweep_inst {
   int N;                 // grid size
   int R;                 // band height (rows)
-  int OC;                // output columns per strip
-  int nstrips, nbands;
-  int ntiles;            // nstrips * nbands: workgroups that share this state per step
+  int OC;                // output columns per wide strip
+  int nstrips, nbands;   // strips (wide + narrow), bands per strip
+  int nwide;             // wide strips: tiles 0 .. nwide*nbands-1
+  int split;             // 1: a narrow last strip, (nbands+1)/2 paired tiles
+  int ntiles;            // workgroups that share this state per step
   double coef;           // (alpha*(N-1))*(N-1)
   double *y0, *y1;       // the two y buffers
   state *st;
@@ -200,6 +208,13 @@ int sweep_band_rows(int pattern, int s, int rows);
 // reflection about row 0 / N-1 (the kernel's fast form); false for grids
 // shorter than the halo, prefetch and ring: those launch the general form.
 bool sweep_single_reflect(int pattern, int s, int N);
+// The same for one instance's tiles: the paired bands of a narrow strip both
+// run the full band height, so the last of them may load rows further past
+// N-1 than a band of its own height.
+bool sweep_inst_single_reflect(int pattern, int s, const sweep_inst &I);
+// Row iterations of tile t of an instance (its band, or the shorter last
+// band, plus the 2(s-1) halo rows; paired bands run the full band height).
+int sweep_tile_iters(int pattern, int s, const sweep_inst &I, int t);
 sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
                            const dev_coef *cf, double *y0, double *y1);
 // tiles == nullptr: a single instance (insts[0]) with ntiles tiles; gen: the

@@ -38,6 +38,31 @@ constexpr double kTileCost[4] = {1.36, 1.0, 1.67, 0.97};
 
 namespace {
 
+// Strips of an N-column grid for strips of 64K lanes: as many full-width
+// strips (64K - 2s output columns) as fit, and the columns left over either
+// in a narrow strip of 32K lanes whose bands are paired two to a workgroup
+// (K even, and they fit its 32K - 2s output columns), or else in one more
+// strip, the strips then evened out.
+struct strip_geom {
+  int nwide;      // full-width strips
+  int oc;         // output columns per full-width strip
+  int split;      // 1: a narrow last strip of paired bands
+  // workgroups for nb bands per strip
+  long long wgs(long long nb) const { return nwide * nb + split * ((nb + 1) / 2); }
+};
+
+strip_geom strips_of(int N, int s, int K)
+{
+  const int oc_max = 64 * K - 2 * s;
+  const int oc_half = (K % 2 == 0) ? 32 * K - 2 * s : 0;
+  const int nfull = N / oc_max;
+  const int rem = N - nfull * oc_max;
+  if (rem > 0 && rem <= oc_half)
+    return strip_geom{nfull, oc_max, 1};
+  const int nstrips = (N + oc_max - 1) / oc_max;
+  return strip_geom{nstrips, (N + nstrips - 1) / nstrips, 0};
+}
+
 // Relative cost of one step of an N x N instance with strips of 64K columns
 // and bands of R rows: lanes x row iterations (the balanced band height plus
 // the 2(s-1) halo iterations and a prologue of about 4) x the measured
@@ -49,14 +74,23 @@ double tile_work(int N, int s, int K, int R, int trip)
   const int oc = L - 2 * s;
   if (oc < 1 || R < 1 || N < 1)
     return 1e300;
-  const long long nstrips = (N + oc - 1) / oc;
+  const strip_geom g = strips_of(N, s, K);
   const long long nbands = (N + R - 1) / R;
-  const long long reff = (N + nbands - 1) / nbands;
+  const long long reff = g.split ? R : (N + nbands - 1) / nbands;
   long long iters = reff + 2 * (s - 1);
   if (trip > 0)
     iters = (iters + trip - 1) / trip * trip;
-  return kTileCost[K - 1] * static_cast<double>(nstrips * L) * static_cast<double>(nbands) *
+  return kTileCost[K - 1] * static_cast<double>(g.wgs(nbands) * L) *
          static_cast<double>(iters + 4);
+}
+
+// Most bands per strip whose workgroups fit `slots`.
+long long bands_for_slots(const strip_geom &g, long long slots)
+{
+  long long nb = static_cast<long long>(static_cast<double>(slots) / (g.nwide + 0.5 * g.split));
+  while (nb > 1 && g.wgs(nb) > slots)
+    --nb;
+  return nb < 1 ? 1 : nb;
 }
 
 }  // namespace
@@ -82,6 +116,24 @@ bool full_trips_of(int pattern)
 bool sweep_single_reflect(int pattern, int s, int N)
 {
   return N > s + prefetch_rows(s) + 3 + (full_trips_of(pattern) ? s + 1 : 0);
+}
+
+bool sweep_inst_single_reflect(int pattern, int s, const sweep_inst &I)
+{
+  if (!sweep_single_reflect(pattern, s, I.N))
+    return false;
+  // the last paired band loads rows up to nbands*R + s - 1 + prefetch
+  return !I.split ||
+         static_cast<long long>(I.nbands) * I.R + s - 1 + prefetch_rows(s) <= 2LL * (I.N - 1);
+}
+
+int sweep_tile_iters(int pattern, int s, const sweep_inst &I, int t)
+{
+  if (t >= I.nwide * I.nbands)
+    return I.R + 2 * (s - 1);                   // paired: the full band height
+  const int band = t % I.nbands;
+  const int rows = std::min(I.R, I.N - band * I.R);
+  return sweep_band_rows(pattern, s, rows) + 2 * (s - 1);
 }
 
 int sweep_band_rows(int pattern, int s, int rows)
@@ -134,13 +186,13 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     const int oc = 64 * K - 2 * s;
     if (oc < 1)
       continue;
-    const int nstrips = (N + oc - 1) / oc;
+    const strip_geom g = strips_of(N, s, K);
     const int res = resident_blocks(pat, K);
     if (res < 1)
       continue;                                     // too wide for this table's kernel
     const int per_cu_min = std::min(res, (4 + K - 1) / K);   // 2K waves each: 2 per SIMD
     for (int per_cu = per_cu_min; per_cu <= res; ++per_cu) {
-      const int rows = band_rows_for(static_cast<long long>(cus) * per_cu / nstrips);
+      const int rows = band_rows_for(bands_for_slots(g, static_cast<long long>(cus) * per_cu));
       const double cost = tile_work(N, s, K, rows, trip);
       if (cost < best_cost) {
         best_cost = cost;
@@ -149,20 +201,19 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     }
   }
   // several rounds for grids too wide for one band per strip to fill the CUs
-  const int oc = 64 * best.waves - 2 * s;
-  const long long nstrips = (N + oc - 1) / oc;
+  const strip_geom g = strips_of(N, s, best.waves);
   auto round_time = [&](int rows) {
-    const long long tiles = nstrips * ((N + rows - 1) / rows);
+    const long long tiles = g.wgs((N + rows - 1) / rows);
     long long iters = rows + 2 * (s - 1);
     if (trip > 0)
       iters = (iters + trip - 1) / trip * trip;
     return static_cast<double>((tiles + cus - 1) / cus) * static_cast<double>(iters + 4);
   };
-  const long long tiles = nstrips * ((N + best.rows - 1) / best.rows);
+  const long long tiles = g.wgs((N + best.rows - 1) / best.rows);
   if (tiles < cus || tiles > static_cast<long long>(cus) * resident_blocks(pat, best.waves)) {
     double best_time = round_time(best.rows);
     for (int k = 2; k <= 16; ++k) {
-      const int rows = band_rows_for(static_cast<long long>(k) * cus / nstrips);
+      const int rows = band_rows_for(bands_for_slots(g, static_cast<long long>(k) * cus));
       const double t = round_time(rows);
       if (t < best_time) {
         best_time = t;
@@ -178,14 +229,15 @@ sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cf
                            const dev_coef *cf, double *y0, double *y1)
 {
   sweep_inst I{};
-  const int L = 64 * c.waves;
-  const int oc_max = L - 2 * m.s;
+  const strip_geom g = strips_of(p.N, m.s, c.waves);
   I.N = p.N;
   I.R = sweep_band_rows(pattern_of(m), m.s, c.rows);
-  I.nstrips = (p.N + oc_max - 1) / oc_max;
-  I.OC = (p.N + I.nstrips - 1) / I.nstrips;
+  I.OC = g.oc;
+  I.nwide = g.nwide;
+  I.split = g.split;
+  I.nstrips = g.nwide + g.split;
   I.nbands = (p.N + I.R - 1) / I.R;
-  I.ntiles = I.nstrips * I.nbands;
+  I.ntiles = static_cast<int>(g.wgs(I.nbands));
   I.coef = p.coef;
   I.y0 = y0;
   I.y1 = y1;

@@ -148,6 +148,7 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
   sweep_cfg cfg{1, 1};
   const int pattern = sweep_pattern(*m);
   int ntiles = 0;
+  bool gen_rows = false;   // the general ghost-row form (sweep_inst_single_reflect)
   sweep_inst si{};
   chain_inst ci{};
   if (chain) {
@@ -165,6 +166,7 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
                               "(1..4, DOPRI87 1..2, wider than 2s), tile_rows >= 1");
     si = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
     ntiles = si.ntiles;
+    gen_rows = !sweep_inst_single_reflect(pattern, s, si);
   }
   // everything the kernels read is written before the first launch
   if (y0_dev != y_dev)
@@ -184,8 +186,8 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
     if (chain)
       return launch_chain(pattern, static_cast<const chain_inst *>(inst_dev), nullptr, ntiles,
                           stream);
-    return launch_sweep(pattern, cfg.waves, !sweep_single_reflect(pattern, s, dp.N),
-                        static_cast<const sweep_inst *>(inst_dev), nullptr, ntiles, stream);
+    return launch_sweep(pattern, cfg.waves, gen_rows, static_cast<const sweep_inst *>(inst_dev),
+                        nullptr, ntiles, stream);
   };
 
   const auto t_start = std::chrono::steady_clock::now();

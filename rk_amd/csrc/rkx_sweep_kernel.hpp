@@ -5,10 +5,12 @@
 // the access distance d = 2N (bruss2d-mix.c:718-721).
 //
 // Work decomposition.  A workgroup owns a strip of L = 64*K grid columns and a
-// band of R rows.  It has 2K wavefronts: waves 0..K-1 carry the u component of
-// the strip's columns (one lane per column), waves K..2K-1 the v component of
-// the same columns, so each wave evaluates one of the two reaction formulas
-// without divergence and holds half of a grid point's accumulators.  The
+// band of R rows (or two bands of a narrow strip of L/2 lanes, one on each
+// half of its waves).  It has 2K wavefronts: waves 0..K-1 carry the u
+// component of the strip's columns (one lane per column), waves K..2K-1 the v
+// component of the same columns, so each wave evaluates one of the two
+// reaction formulas without divergence and holds half of a grid point's
+// accumulators.  The
 // workgroup sweeps the band's rows in the block-diagonal order of PipeD: at
 // row iteration r, stage i is applied to row r - i, so when stage i needs w_i
 // on rows k-1, k, k+1, row k+1 has just been completed by stage i-1 in the
@@ -100,15 +102,43 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   const double coef = I.coef;
   const int R = I.R, OC = I.OC;
   const dev_coef *__restrict__ cf = I.cf;
-  const int strip = td.y / I.nbands;
-  const int band = td.y - strip * I.nbands;
-
   const int var = (threadIdx.x >= L) ? 1 : 0;   // wave-uniform: 0 = u, 1 = v
   const int cl = threadIdx.x - var * L;         // column lane within the strip
-  const int col = strip * OC - s + cl;
+
+  // The tile's columns and band.  A wide tile is one band of a strip of L
+  // lanes.  A paired tile carries two bands of the narrow last strip: lanes
+  // [0, L/2) of each component the band 2p, lanes [L/2, L) the band 2p+1
+  // (whole waves, as L/2 is a multiple of 64); without a band 2p+1 those
+  // waves repeat band 2p and store nothing.  Both halves run the full band
+  // height, so every wave passes the same barriers.
+  const int nwt = I.nwide * I.nbands;
+  const bool paired = td.y >= nwt;
+  int band, lo, lsub, c0, c1;
+  bool live = true;
+  if (!paired) {
+    const int strip = td.y / I.nbands;
+    band = td.y - strip * I.nbands;
+    lo = 0;
+    lsub = L;
+    c0 = strip * OC;
+    c1 = c0 + OC;
+  } else {
+    const int half = __builtin_amdgcn_readfirstlane(cl >= L / 2 ? 1 : 0);
+    band = 2 * (td.y - nwt) + half;
+    if (band >= I.nbands) {
+      band = I.nbands - 1;
+      live = false;
+    }
+    lo = half * (L / 2);
+    lsub = L / 2;
+    c0 = I.nwide * OC;
+    c1 = N;
+  }
+  const int cs = cl - lo;                       // column lane within the tile
+  const int col = c0 - s + cs;
   const int ccol = reflect(col, N);             // in-bounds column for loads
-  const bool out_col = (cl >= s) && (cl < L - s) && (col >= 0) && (col < N) &&
-                       (col < (strip + 1) * OC);
+  const bool out_col = live && (cs >= s) && (cs < lsub - s) && (col >= 0) && (col < N) &&
+                       (col < c1);
   // Horizontal part of the Laplacian: ((vN + vS) + X) + m*Y.  Interior
   // columns read X = W, Y = E, m = 1.  Column 0 reads X from the zero slot and
   // Y = E with m = 2; column N-1 reads X from the zero slot and Y = W with
@@ -126,7 +156,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   const int r0 = band * R;
   const int r1 = min(r0 + R, N);
   const int it0 = r0 - (s - 1);                  // first row iteration
-  const int it1 = r1 + s - 1;                    // one past the last
+  const int it1 = paired ? it0 + R + 2 * (s - 1) : r1 + s - 1;   // one past the last
 
   // register ring: slot -> [0] dy, [1..s-1] w_l, [s] err (this lane's component)
   double acc[NS][s + 1];
@@ -219,8 +249,9 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
         // Every stage runs every iteration: during the first 2(s-1)
         // iterations of a band, stage i works on rows above its range, whose
         // results are never read by an in-range row and never stored; with
-        // full trips, iterations past it1 work on rows below the band, which
-        // are never stored either.
+        // full trips, and in a paired band shorter than R, the last
+        // iterations work on rows below the band, which are never stored
+        // either.
         const int buf = (rr - it0) & 1;
         __syncthreads();
         double nX[s], nY[s], nP[s];
@@ -290,7 +321,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
           if constexpr (i == s - 1) {
             // y_new = y + dy; the error term against y_old (block-inline.h:298-320)
             const double ynw = yold + acc[sl][0];
-            const bool out = out_col && (kl >= r0) && (!P::full_trips || kl < r1);
+            const bool out = out_col && (kl >= r0) && (kl < r1);
             // Straight-line code, so the iteration stays one scheduling
             // region: the store is masked by buffer range checking, and the
             // division runs in every lane on selected operands.  q = |e| /

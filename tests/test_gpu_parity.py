@@ -143,6 +143,30 @@ def test_c1_host_control_bitwise(rk, orc, impl, waves, rows):
     assert np.array_equal(y, yo)
 
 
+# Grids whose columns past the full-width strips fit a strip of half the lanes:
+# the fused kernel pairs two bands of that narrow strip in one workgroup.
+# (N, waves, rows, table, alpha): odd band counts (one half without a band),
+# ragged last bands, no full-width strip at all (N = 60), whole-trip tables.
+PAIRED = [(130, 2, 9, "DOPRI54", 2e-3), (260, 4, 37, "DOPRI54", 2e-3),
+          (280, 4, 50, "VERNER65", 2e-3), (60, 4, 11, "RKF23", 2e-3),
+          (130, 2, 20, "DOPRI87", 3e-3)]
+
+
+@pytest.mark.parametrize("N,waves,rows,meth,alpha", PAIRED)
+def test_paired_narrow_strip_bitwise(rk, orc, N, waves, rows, meth, alpha):
+    po = orc.make_ode("bruss2d", N, alpha)
+    y0 = orc.ode_start(po)
+    tol = 1e-9 if meth == "DOPRI87" else 1e-6
+    yo, so, tro = orc.solve(po, meth, 0.0, 0.3, y0, tol, h0=1e-4, trace_cap=2000)
+    p = rk.Bruss2D(N, alpha)
+    y, st, tr = run_gpu(rk, p, meth, 0.0, 0.3, y0, tol, h0=1e-4, impl="PipeDls", ctrl="host",
+                        tile_waves=waves, tile_rows=rows, trace_cap=2000)
+    assert (st.steps_acc, st.steps_rej) == (so["acc"], so["rej"])
+    assert st.steps_acc > 5
+    assert np.array_equal(tr, tro)
+    assert np.array_equal(y, yo)
+
+
 @pytest.mark.parametrize("name", sorted(DUMPS))
 @pytest.mark.parametrize("impl", ["D", "PipeDls"])
 def test_golden_dump_host_control(rk, name, impl):
