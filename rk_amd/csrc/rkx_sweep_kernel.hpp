@@ -228,14 +228,14 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
       constexpr int i = decltype(itg)::value;
       xch[0][i][VAR][cl] = (i == 0) ? yr[0] : 0.0;
     });
-    for (int rb = it0; rb < it1; rb += NS) {
-      static_for<NS>([&](auto ut) {
+    // One row iteration rr; U = (rr - it0) % NS names the ring slots.  Stage
+    // i runs when 2i <= J: in the band's first 2(s-1) iterations (J = the
+    // iteration's index) stage i would work on rows above the ones later
+    // stages need (row rr - i < r0 - (s-1-i)), whose values reach no stored
+    // row, so it is left out; J = 2(s-1) runs every stage.
+    auto iter = [&](const int rr, auto ut, auto jt) {
         constexpr int U = decltype(ut)::value;
-        const int rr = rb + U;
-        if constexpr (!P::full_trips) {
-          if (rr >= it1)
-            return;
-        }
+        constexpr int J = decltype(jt)::value;
         // Loads complete in issue order, so every load used in this
         // iteration was issued in an earlier one: y for stage 0 PF rows
         // ahead, and y_old of the row the last stage finishes DY iterations
@@ -246,20 +246,21 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
         const int kl = rr - (s - 1);                 // row finishing this iteration
         const double yold = yo[U % NS];
 
-        // Every stage runs every iteration: during the first 2(s-1)
-        // iterations of a band, stage i works on rows above its range, whose
-        // results are never read by an in-range row and never stored; with
-        // full trips, and in a paired band shorter than R, the last
-        // iterations work on rows below the band, which are never stored
-        // either.
+        // With full trips the first 2(s-1) iterations run every stage (their
+        // extra work on rows above the range is never read by an in-range
+        // row and never stored); with full trips, and in a paired band
+        // shorter than R, the last iterations work on rows below the band,
+        // which are never stored either.
         const int buf = (rr - it0) & 1;
         __syncthreads();
         double nX[s], nY[s], nP[s];
         static_for<s>([&](auto itg) {
           constexpr int i = decltype(itg)::value;
-          nX[i] = xch[buf][i][VAR][slotX];
-          nY[i] = xch[buf][i][VAR][slotY];
-          nP[i] = xch[buf][i][VAR ^ 1][cl];           // the other component
+          if constexpr (2 * i <= J) {
+            nX[i] = xch[buf][i][VAR][slotX];
+            nY[i] = xch[buf][i][VAR][slotY];
+            nP[i] = xch[buf][i][VAR ^ 1][cl];           // the other component
+          }
         });
         // The next iteration's exchange goes to the other buffer as each
         // value becomes final: its stage-0 row now, w_{i+1} of row rr - i
@@ -269,6 +270,8 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
 
         static_for<s>([&](auto itg) {
           constexpr int i = decltype(itg)::value;
+          if constexpr (2 * i > J)
+            return;
           constexpr int sl = (U - i + NS) % NS;              // slot of row rr - i
           constexpr int sn = (U - i - 1 + 2 * NS) % NS;      // row rr - i - 1
           constexpr int ss = (U - i + 1 + NS) % NS;          // row rr - i + 1
@@ -344,6 +347,25 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
             qmax = fmax(qmax, qv);
           }
         });
+    };
+    constexpr int JALL = 2 * (s - 1);
+    // the first 2(s-1) iterations peeled (every band has more: R >= 1);
+    // tables that run whole trips keep them in the loop
+    constexpr int PRO = P::full_trips ? 0 : JALL;
+    static_for<PRO>([&](auto jt) {
+      constexpr int j = decltype(jt)::value;
+      iter(it0 + j, std::integral_constant<int, j % NS>{}, jt);
+    });
+    for (int rb = it0 + PRO; rb < it1; rb += NS) {
+      static_for<NS>([&](auto ut) {
+        constexpr int U = decltype(ut)::value;
+        const int rr = rb + U;
+        if constexpr (!P::full_trips) {
+          if (rr >= it1)
+            return;
+        }
+        iter(rr, std::integral_constant<int, (U + PRO) % NS>{},
+             std::integral_constant<int, JALL>{});
       });
     }
   };
