@@ -202,7 +202,8 @@ int sweep_pattern(const rkx_method &m);   // compiled zero pattern (0: none)
 sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m);
 // Band height a tile gets for `rows` requested rows: unchanged, or for tables
 // whose fused kernel runs whole trips of its s+1-row ring, rounded up so that
-// the band's rows + 2(s-1) row iterations fill whole trips.
+// the band's rows + 2(s-1) row iterations, less the peeled ones
+// (peeled_iters), fill whole trips.
 int sweep_band_rows(int pattern, int s, int rows);
 // Whether every ghost row a band of an N-row grid loads lies within a single
 // reflection about row 0 / N-1 (the kernel's fast form); false for grids

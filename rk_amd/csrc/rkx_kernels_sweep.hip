@@ -67,8 +67,8 @@ strip_geom strips_of(int N, int s, int K)
 // and bands of R rows: lanes x row iterations (the balanced band height plus
 // the 2(s-1) halo iterations and a prologue of about 4) x the measured
 // per-lane factor of the strip width.  trip > 0 rounds a band's row
-// iterations up to whole trips of that many rows.
-double tile_work(int N, int s, int K, int R, int trip)
+// iterations after the pro peeled ones up to whole trips of that many rows.
+double tile_work(int N, int s, int K, int R, int trip, int pro)
 {
   const int L = 64 * K;
   const int oc = L - 2 * s;
@@ -79,7 +79,7 @@ double tile_work(int N, int s, int K, int R, int trip)
   const long long reff = g.split ? R : (N + nbands - 1) / nbands;
   long long iters = reff + 2 * (s - 1);
   if (trip > 0)
-    iters = (iters + trip - 1) / trip * trip;
+    iters = pro + (iters - pro + trip - 1) / trip * trip;
   return kTileCost[K - 1] * static_cast<double>(g.wgs(nbands) * L) *
          static_cast<double>(iters + 4);
 }
@@ -141,8 +141,9 @@ int sweep_band_rows(int pattern, int s, int rows)
   if (!full_trips_of(pattern))
     return rows;
   const int ns = s + 1;
-  const int it = rows + 2 * (s - 1);
-  return (it + ns - 1) / ns * ns - 2 * (s - 1);
+  const int pro = peeled_iters(true, s);
+  const int it = rows + 2 * (s - 1) - pro;      // iterations in the loop
+  return (it + ns - 1) / ns * ns - 2 * (s - 1) + pro;
 }
 
 // Default tile: strips of 64*K columns (2K waves per workgroup) and the band
@@ -167,6 +168,7 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
   const int N = p.N;
   const int pat = pattern_of(m);
   const int trip = full_trips_of(pat) ? s + 1 : 0;
+  const int pro = peeled_iters(true, s);      // used with trip only
   int cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -193,7 +195,7 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     const int per_cu_min = std::min(res, (4 + K - 1) / K);   // 2K waves each: 2 per SIMD
     for (int per_cu = per_cu_min; per_cu <= res; ++per_cu) {
       const int rows = band_rows_for(bands_for_slots(g, static_cast<long long>(cus) * per_cu));
-      const double cost = tile_work(N, s, K, rows, trip);
+      const double cost = tile_work(N, s, K, rows, trip, pro);
       if (cost < best_cost) {
         best_cost = cost;
         best = sweep_cfg{K, rows};
@@ -206,7 +208,7 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
     const long long tiles = g.wgs((N + rows - 1) / rows);
     long long iters = rows + 2 * (s - 1);
     if (trip > 0)
-      iters = (iters + trip - 1) / trip * trip;
+      iters = pro + (iters - pro + trip - 1) / trip * trip;
     return static_cast<double>((tiles + cus - 1) / cus) * static_cast<double>(iters + 4);
   };
   const long long tiles = g.wgs((N + best.rows - 1) / best.rows);

@@ -81,6 +81,13 @@ struct pat_ssprk3 {
 // y rows the fused kernel loads ahead of its first stage (bounded by the ring)
 constexpr int prefetch_rows(int s) { return (s - 2 < 2) ? s - 2 : 2; }
 
+// Row iterations at the start of a band that the fused kernel runs outside
+// its loop (the first 2(s-1), in which the stages whose rows no stored row
+// needs are left out); none for the 13-stage tables, whose peeled code would
+// be too long.  Tables whose kernel runs whole trips of the ring then fill
+// whole trips with the rest of the band.
+constexpr int peeled_iters(bool full_trips, int s) { return (full_trips && s > 9) ? 0 : 2 * (s - 1); }
+
 // compile-time loop: f(integral_constant<int, I>) for I = 0 .. N-1, so ring
 // slots and stage indices are constants and the register ring stays in VGPRs
 template <class F, int... Is>

@@ -349,9 +349,9 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
         });
     };
     constexpr int JALL = 2 * (s - 1);
-    // the first 2(s-1) iterations peeled (every band has more: R >= 1);
-    // tables that run whole trips keep them in the loop
-    constexpr int PRO = P::full_trips ? 0 : JALL;
+    // the peeled iterations (every band runs more: R >= 1); with whole
+    // trips the band height makes the rest fill whole trips
+    constexpr int PRO = peeled_iters(P::full_trips, s);
     static_for<PRO>([&](auto jt) {
       constexpr int j = decltype(jt)::value;
       iter(it0 + j, std::integral_constant<int, j % NS>{}, jt);
