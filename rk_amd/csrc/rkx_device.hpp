@@ -238,9 +238,12 @@ __device__ __forceinline__ double wave_max(double v)
 // Folds each thread's (qmax, zero) into the step accumulators and runs the
 // control in the last of the `nblocks` workgroups that work on this state to
 // finish (nblocks = 0: the whole grid).  Must be called by all threads of the
-// block.  red must hold blockDim.x/64 doubles of LDS.
+// block.  red must hold blockDim.x/64 doubles of LDS.  coef = false: the
+// kernels of this state form h * dev_method::col themselves, so the control
+// writes no premultiplied coefficients.
 __device__ inline void block_finish(state *st, double qmax, unsigned int zero, double *red,
-                                    unsigned int *zred, unsigned int nblocks = 0u)
+                                    unsigned int *zred, unsigned int nblocks = 0u,
+                                    bool coef = true)
 {
   if (nblocks == 0u)
     nblocks = gridDim.x * gridDim.y * gridDim.z;
@@ -279,7 +282,7 @@ __device__ inline void block_finish(state *st, double qmax, unsigned int zero, d
       hn = st->h;
     }
     __syncthreads();
-    if (pm)
+    if (pm && coef)
       premult_block(*st->meth, st->s, hn, *st->coef, static_cast<int>(threadIdx.x),
                     static_cast<int>(blockDim.x));
   }

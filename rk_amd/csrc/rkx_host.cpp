@@ -115,6 +115,12 @@ void fill_dev_method(const rkx_method &m, dev_method &d)
     d.e[i] = m.b[i] - m.b_hat[i];      // seq/D.c:40-41
     d.c[i] = m.c[i];
   }
+  for (int i = 0; i < m.s; ++i) {
+    for (int l = i + 1; l < m.s; ++l)
+      d.col[i][l - i - 1] = d.A[l * RKX_MAX_STAGES + i];
+    d.col[i][m.s - i - 1] = d.b[i];
+    d.col[i][m.s - i] = d.e[i];
+  }
 }
 
 // initial_stepsize (src/tools.c:25-36)

@@ -44,6 +44,11 @@ struct dev_method {
   double b[RKX_MAX_STAGES];
   double e[RKX_MAX_STAGES];
   double c[RKX_MAX_STAGES];
+  // grouped by the stage that scatters with them, as dev_coef::col but not
+  // multiplied by h: col[i] = {A[i+1][i], ..., A[s-1][i], b[i], e[i]}.  The
+  // fused kernels form h * col themselves at their start (the same products
+  // as premult), so their steps need no premultiplication in the control.
+  double col[RKX_MAX_STAGES][RKX_MAX_STAGES + 2];
 };
 
 // Premultiplied coefficients of the current step (premult, src/methods.c:43-58):

@@ -62,23 +62,25 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
   p.d1 = I.d1;
   p.c1 = I.c1;
   const double D = I.diff;
-  const dev_coef *__restrict__ cf = I.cf;
+  const dev_method *__restrict__ mt = st->meth;
+  const double h = st->h;                        // this step's h (uniform)
 
   const int t = threadIdx.x;
 
+  // h * col[i]: the premultiplied coefficients of stage i (premult's products)
   double kv[s][s + 1];
   static_for<s>([&](auto it) {
     constexpr int i = decltype(it)::value;
     static_for<s>([&](auto lt) {
       constexpr int l = decltype(lt)::value;
       if constexpr (l > i)
-        kv[i][l - i - 1] = nzA<P>(l, i) ? wave_uniform(cf->col[i][l - i - 1]) : 0.0;
+        kv[i][l - i - 1] = nzA<P>(l, i) ? wave_uniform(h * mt->col[i][l - i - 1]) : 0.0;
     });
     if constexpr (P::fsal && i < s - 1)
       kv[i][s - i - 1] = kv[i][s - i - 2];
     else
-      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? wave_uniform(cf->col[i][s - i - 1]) : 0.0;
-    kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(cf->col[i][s - i]) : 0.0;
+      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? wave_uniform(h * mt->col[i][s - i - 1]) : 0.0;
+    kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(h * mt->col[i][s - i]) : 0.0;
   });
 
   double qmax = 0.0;
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
   });
   __syncthreads();                       // the next segment reuses the edge buffers
   }
-  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.nwg));
+  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.nwg), false);
 }
 
 }  // namespace

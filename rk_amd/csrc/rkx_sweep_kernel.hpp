@@ -101,7 +101,8 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   const int N2 = N + N;
   const double coef = I.coef;
   const int R = I.R, OC = I.OC;
-  const dev_coef *__restrict__ cf = I.cf;
+  const dev_method *__restrict__ mt = st->meth;
+  const double h = st->h;                        // this step's h (uniform)
   const int var = (threadIdx.x >= L) ? 1 : 0;   // wave-uniform: 0 = u, 1 = v
   const int cl = threadIdx.x - var * L;         // column lane within the strip
 
@@ -195,24 +196,25 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
     static_for<s + 1>([&](auto et) { acc[d][decltype(et)::value] = 0.0; });
   });
 
-  // The step's premultiplied coefficients, loaded once (uniform, SGPRs):
-  // kv[i] = col[i] = {hA[i+1..s-1][i], hb[i], he[i]} for the scatter of stage
-  // i.  Only the non-zero entries are loaded; for a first-same-as-last table
-  // (A[s-1][j] == b[j], e.g. DOPRI54) h*A[s-1][j] and h*b[j] are the same
-  // product, so hb[j] reuses the register of hA[s-1][j].
+  // The step's premultiplied coefficients, formed once (uniform, SGPRs):
+  // kv[i] = h * col[i] = {hA[i+1..s-1][i], hb[i], he[i]} for the scatter of
+  // stage i, the products premult forms (src/methods.c:43-58).  Only the
+  // non-zero entries are formed; for a first-same-as-last table (A[s-1][j]
+  // == b[j], e.g. DOPRI54) h*A[s-1][j] and h*b[j] are the same product, so
+  // hb[j] reuses the register of hA[s-1][j].
   double kv[s][s + 1];
   static_for<s>([&](auto it) {
     constexpr int i = decltype(it)::value;
     static_for<s>([&](auto lt) {
       constexpr int l = decltype(lt)::value;
       if constexpr (l > i)
-        kv[i][l - i - 1] = nzA<P>(l, i) ? wave_uniform(cf->col[i][l - i - 1]) : 0.0;
+        kv[i][l - i - 1] = nzA<P>(l, i) ? wave_uniform(h * mt->col[i][l - i - 1]) : 0.0;
     });
     if constexpr (P::fsal && i < s - 1)
       kv[i][s - i - 1] = kv[i][s - i - 2];
     else
-      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? wave_uniform(cf->col[i][s - i - 1]) : 0.0;
-    kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(cf->col[i][s - i]) : 0.0;
+      kv[i][s - i - 1] = ((P::b >> i) & 1u) ? wave_uniform(h * mt->col[i][s - i - 1]) : 0.0;
+    kv[i][s - i] = ((P::e >> i) & 1u) ? wave_uniform(h * mt->col[i][s - i]) : 0.0;
   });
 
   double qmax = 0.0;
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
     rows(std::integral_constant<int, 0>{});
   else
     rows(std::integral_constant<int, 1>{});
-  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.ntiles));
+  block_finish(st, qmax, zero, red, zred, static_cast<unsigned int>(I.ntiles), false);
 }
 
 template <class P, int K>
