@@ -129,9 +129,13 @@ typedef struct rkx_opts {
   int impl;              /* RKX_IMPL_* */
   int ctrl;              /* RKX_CTRL_* */
   int tile_rows;         /* sweep band height in grid rows (0: tuned default; VERNER65
-                            rounds it up so that a band fills whole trips of its ring) */
+                            rounds it up so that a band's row iterations after its
+                            first 2(s-1) fill whole trips of its ring) */
   int tile_waves;        /* sweep strip width in wavefronts per component, 1..4 (DOPRI87:
-                            1..2, its kernel runs one wave per SIMD); 0: tuned default */
+                            1..2, its kernel runs one wave per SIMD); 0: tuned default.
+                            With an even width, columns left over after the full
+                            strips that fit half a strip run as a narrow strip whose
+                            bands share workgroups two at a time */
   double *trace;         /* optional host buffer, 3 doubles (t, h, err) per attempt */
   int trace_cap;         /* attempts the trace buffer holds */
   void *stream;          /* hipStream_t to run on; NULL: the library's own stream */
