@@ -64,6 +64,29 @@ def test_short_and_tall_grids_share_a_launch(rk, meth, grids):
             assert np.array_equal(b.result(i), y_alone)
 
 
+@pytest.mark.parametrize("waves,rows", [(4, 23), (2, 9)])
+def test_paired_narrow_strips_in_a_batch(rk, waves, rows):
+    """Grids whose columns left over after the full-width strips run as a
+    narrow strip of paired bands (N = 40, 60, 260 at 4 waves; 40, 130, 242,
+    260 at 2) share launches with grids that have none (130, 200, 242 at 4
+    waves; 60, 200 at 2); the instances end at different times, so the batch
+    shortens the bands of the ones left while it runs.  Every instance
+    equals its alone run bitwise."""
+    from rk_amd.workloads import bruss_instance
+    grids = [(60, 0.3), (130, 0.5), (200, 0.2), (242, 0.4), (260, 0.6), (40, 0.25)]
+    insts = [bruss_instance(N, 2e-3, k, seed=6) for k, (N, _) in enumerate(grids)]
+    with rk.Batch(tile_waves=waves, tile_rows=rows) as b:
+        ids = [b.add(p, "DOPRI54", 0.0, te, y0, 1e-6, h0=1e-3)
+               for (p, y0), (_, te) in zip(insts, grids)]
+        assert b.run(100000) == 0
+        for (p, y0), (_, te), i in zip(insts, grids, ids):
+            y_alone, st_alone = plain(rk, p, y0, "DOPRI54", te, 1e-6, 1e-3)
+            st, done = b.status(i)
+            assert done
+            assert (st.steps_acc, st.steps_rej) == (st_alone.steps_acc, st_alone.steps_rej)
+            assert np.array_equal(b.result(i), y_alone)
+
+
 def test_dopri87_instances_in_a_batch(rk):
     """DOPRI87 instances run their fused kernel in a batch (strip width
     narrowed to the 2 waves it is built for when the options ask for 4) and
