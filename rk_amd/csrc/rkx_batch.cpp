@@ -135,7 +135,7 @@ int adapt_bands(rkx_batch *b)
       if (!I.used || I.done || I.storage != RKX_STORAGE_LOW || I.K <= 0)
         continue;
       const sweep_cfg c{I.base.waves, rows_at(I, div)};
-      const sweep_inst si = make_sweep_inst(I.dp, I.m, c, nullptr, nullptr, nullptr, nullptr);
+      const sweep_inst si = make_sweep_inst(I.dp, I.m, c, nullptr, nullptr, nullptr);
       const int band = sweep_band_rows(I.pattern, I.m.s, std::min(si.R, I.dp.N));
       const double iters = band + 2.0 * (I.m.s - 1);
       auto &e = g[{I.pattern, I.K}];
@@ -167,8 +167,7 @@ int adapt_bands(rkx_batch *b)
     if (c.rows == I.cfg.rows)
       continue;
     I.cfg = c;
-    const sweep_inst si = make_sweep_inst(I.dp, I.m, c, b->st_dev + i, b->cf_dev + i, I.y[0],
-                                          I.y[1]);
+    const sweep_inst si = make_sweep_inst(I.dp, I.m, c, b->st_dev + i, I.y[0], I.y[1]);
     RKX_HIP(hipMemcpyAsync(b->si_dev + i, &si, sizeof(si), hipMemcpyHostToDevice, b->stream));
   }
   return RKX_OK;
@@ -178,7 +177,7 @@ int adapt_bands(rkx_batch *b)
 // last band, plus the 2(s-1) halo rows).
 int tile_iters(const batch_inst &I, int t)
 {
-  const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
+  const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr);
   return sweep_tile_iters(I.pattern, I.m.s, si, t);
 }
 
@@ -219,9 +218,9 @@ int rebuild_tables(rkx_batch *b)
     g.K = I.K;
     int nt = 0;
     if (I.K == 0) {
-      nt = make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr, nullptr).nwg;
+      nt = make_chain_inst(I.dp, I.m, nullptr, nullptr, nullptr).nwg;
     } else {
-      const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr);
+      const sweep_inst si = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr);
       nt = si.ntiles;
       if (!sweep_inst_single_reflect(I.pattern, I.m.s, si))
         g.gen = true;
@@ -554,7 +553,7 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
     I.pattern = sweep_pattern(*m);
     I.K = 0;
     const chain_inst ci =
-        make_chain_inst(dp, *m, b->st_dev + slot, b->cf_dev + slot, I.y[0], I.y[1]);
+        make_chain_inst(dp, *m, b->st_dev + slot, I.y[0], I.y[1]);
     RKX_HIP(hipMemcpyAsync(b->ci_dev + slot, &ci, sizeof(ci), hipMemcpyHostToDevice, b->stream));
   } else if (storage == RKX_STORAGE_LOW) {
     I.pattern = sweep_pattern(*m);
@@ -562,7 +561,7 @@ int rkx_batch_add(rkx_batch *b, const rkx_ode *p, const rkx_method *m, double t0
     I.base = I.cfg;
     I.K = I.cfg.waves;
     const sweep_inst si =
-        make_sweep_inst(dp, *m, I.cfg, b->st_dev + slot, b->cf_dev + slot, I.y[0], I.y[1]);
+        make_sweep_inst(dp, *m, I.cfg, b->st_dev + slot, I.y[0], I.y[1]);
     RKX_HIP(hipMemcpyAsync(b->si_dev + slot, &si, sizeof(si), hipMemcpyHostToDevice, b->stream));
   }
   RKX_HIP(hipMemcpyAsync(b->dm_dev + slot, &I.dm, sizeof(I.dm), hipMemcpyHostToDevice, b->stream));
@@ -679,7 +678,7 @@ int rkx_batch_tile(rkx_batch *b, int id, int *waves, int *rows)
   *rows = 0;
   if (I.storage == RKX_STORAGE_LOW && I.K > 0) {
     *waves = I.cfg.waves;
-    *rows = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr, nullptr).R;
+    *rows = make_sweep_inst(I.dp, I.m, I.cfg, nullptr, nullptr, nullptr).R;
   }
   return RKX_OK;
 }

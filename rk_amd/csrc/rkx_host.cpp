@@ -606,14 +606,14 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   if (chain) {
     if (int rc = mem.alloc(reinterpret_cast<void **>(&chain_dev), sizeof(chain_inst)))
       return rc;
-    const chain_inst hi = make_chain_inst(dp, *m, st_dev, cf_dev, ybuf[0], ybuf[1]);
+    const chain_inst hi = make_chain_inst(dp, *m, st_dev, ybuf[0], ybuf[1]);
     ntiles = hi.nwg;
     RKX_HIP(hipMemcpyAsync(chain_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));
     RKX_HIP(hipStreamSynchronize(stream));
   } else if (sweep) {
     if (int rc = mem.alloc(reinterpret_cast<void **>(&inst_dev), sizeof(sweep_inst)))
       return rc;
-    const sweep_inst hi = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
+    const sweep_inst hi = make_sweep_inst(dp, *m, cfg, st_dev, ybuf[0], ybuf[1]);
     ntiles = hi.ntiles;
     gen_rows = !sweep_inst_single_reflect(pattern, s, hi);
     RKX_HIP(hipMemcpyAsync(inst_dev, &hi, sizeof(hi), hipMemcpyHostToDevice, stream));

@@ -198,8 +198,7 @@ struct sweep_inst {
   int ntiles;            // workgroups that share this state per step
   double coef;           // (alpha*(N-1))*(N-1)
   double *y0, *y1;       // the two y buffers
-  state *st;
-  const dev_coef *cf;
+  state *st;             // step state; the kernel forms h*A etc. from st->meth
 };
 
 bool sweep_supported(const dev_ode &p, const rkx_method &m);
@@ -222,7 +221,7 @@ bool sweep_inst_single_reflect(int pattern, int s, const sweep_inst &I);
 // band, plus the 2(s-1) halo rows; paired bands run the full band height).
 int sweep_tile_iters(int pattern, int s, const sweep_inst &I, int t);
 sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
-                           const dev_coef *cf, double *y0, double *y1);
+                           double *y0, double *y1);
 // tiles == nullptr: a single instance (insts[0]) with ntiles tiles; gen: the
 // general reflection form (some instance fails sweep_single_reflect)
 hipError_t launch_sweep(int pattern, int K, bool gen, const sweep_inst *insts, const int2 *tiles,
@@ -243,13 +242,12 @@ struct chain_inst {
   int nwg;               // workgroups per step (each loops over segments)
   double c0, d0, d1, c1, diff;
   double *y0, *y1;
-  state *st;
-  const dev_coef *cf;
+  state *st;             // step state; the kernel forms h*A etc. from st->meth
 };
 
 bool chain_supported(const dev_ode &p, const rkx_method &m);
-chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, const dev_coef *cf,
-                           double *y0, double *y1);
+chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, double *y0,
+                           double *y1);
 hipError_t launch_chain(int pattern, const chain_inst *insts, const int2 *tiles, int ntiles,
                         hipStream_t stream);
 

@@ -174,8 +174,8 @@ bool chain_supported(const dev_ode &p, const rkx_method &m)
   return p.kind == RKX_LV_CHAIN && pat >= 1 && pat <= 6 && kT > 2 * m.s;
 }
 
-chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, const dev_coef *cf,
-                           double *y0, double *y1)
+chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, double *y0,
+                           double *y1)
 {
   chain_inst I{};
   I.cells = p.cells;
@@ -190,7 +190,6 @@ chain_inst make_chain_inst(const dev_ode &p, const rkx_method &m, state *st, con
   I.y0 = y0;
   I.y1 = y1;
   I.st = st;
-  I.cf = cf;
   return I;
 }
 

@@ -228,7 +228,7 @@ sweep_cfg sweep_default_cfg(const dev_ode &p, const rkx_method &m)
 
 // Geometry of one instance's tiles for strips of 64*K columns and bands of R rows.
 sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cfg &c, state *st,
-                           const dev_coef *cf, double *y0, double *y1)
+                           double *y0, double *y1)
 {
   sweep_inst I{};
   const strip_geom g = strips_of(p.N, m.s, c.waves);
@@ -244,7 +244,6 @@ sweep_inst make_sweep_inst(const dev_ode &p, const rkx_method &m, const sweep_cf
   I.y0 = y0;
   I.y1 = y1;
   I.st = st;
-  I.cf = cf;
   return I;
 }
 

@@ -152,7 +152,7 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
   sweep_inst si{};
   chain_inst ci{};
   if (chain) {
-    ci = make_chain_inst(dp, *m, st_dev, cf_dev, ybuf[0], ybuf[1]);
+    ci = make_chain_inst(dp, *m, st_dev, ybuf[0], ybuf[1]);
     ntiles = ci.nwg;
   } else if (!pairs) {
     cfg = sweep_default_cfg(dp, *m);
@@ -164,7 +164,7 @@ int rkx_solve_noalloc(const rkx_ode *p, const rkx_method *m, double t0, double t
         sweep_resident_blocks(pattern, cfg.waves) < 1)
       return fail(RKX_EINVAL, "rkx_solve_noalloc: tile_waves must fit this table's fused kernel "
                               "(1..4, DOPRI87 1..2, wider than 2s), tile_rows >= 1");
-    si = make_sweep_inst(dp, *m, cfg, st_dev, cf_dev, ybuf[0], ybuf[1]);
+    si = make_sweep_inst(dp, *m, cfg, st_dev, ybuf[0], ybuf[1]);
     ntiles = si.ntiles;
     gen_rows = !sweep_inst_single_reflect(pattern, s, si);
   }
