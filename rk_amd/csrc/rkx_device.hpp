@@ -27,6 +27,29 @@ using gmut = __attribute__((address_space(1))) double;
 __device__ __forceinline__ gconst *as_global(const double *p) { return (gconst *) p; }
 __device__ __forceinline__ gmut *as_global(double *p) { return (gmut *) p; }
 
+// What a step kernel reads of its state at its start.  Read through the
+// global address space, before any store of the kernel, the loads are
+// scalar (through the generic pointer they were vector loads), cur and done
+// in one.  No workgroup of a state reads it after that state's control has
+// run: the control runs in the last of the state's workgroups to finish,
+// after all of them have started.
+struct step_view {
+  int done, cur;
+  double h;
+  const dev_method *meth;
+};
+__device__ __forceinline__ step_view read_step(const state *st)
+{
+  static_assert(offsetof(state, done) == offsetof(state, cur) + 4, "cur, done adjacent");
+  using gstate = const __attribute__((address_space(1))) state;
+  using gu64 = const __attribute__((address_space(1))) unsigned long long;
+  gstate *g = (gstate *) st;
+  const unsigned long long cd = *(gu64 *) &g->cur;     // cur, done in one load
+  return step_view{static_cast<int>(cd >> 32), static_cast<int>(cd & 0xffffffffu), g->h, g->meth};
+}
+using gmethod = const __attribute__((address_space(1))) dev_method;
+__device__ __forceinline__ gmethod *as_global(const dev_method *m) { return (gmethod *) m; }
+
 // Stores v to row[idx] in the lanes where keep holds, without a branch: the
 // store goes through a buffer resource of the row (len doubles); the other
 // lanes address past its end, and the hardware drops out-of-range buffer

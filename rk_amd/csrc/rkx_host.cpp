@@ -680,7 +680,12 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
   } else {
     // Chunks of steps captured once as a graph.  The host keeps two chunks in
     // flight: it enqueues chunk k+1 (and a snapshot copy of the state) before
-    // waiting on the snapshot taken after chunk k.
+    // waiting on the snapshot taken after chunk k.  The snapshot copy leaves
+    // the GPU idle for about 18 us, so a solve that is still running after
+    // its first kChunkGrowAfter chunks launches the graph kChunkReps times
+    // per snapshot (a short solve keeps short chunks: after the last step
+    // every launch in flight returns at once, but still costs its launch).
+    constexpr int kChunkGrowAfter = 8, kChunkReps = 4;
     const int chunk = fused ? 32 : 16;
     graph_holder gh;
     RKX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -704,8 +709,11 @@ int rkx_solve(const rkx_ode *p, const rkx_method *m, double t0, double te,
         (void) hipEventDestroy(e[1]);
       }
     } evf{ev};
+    int launched = 0;
     auto launch_chunk = [&](int slot) -> int {
-      RKX_HIP(hipGraphLaunch(gh.ge, stream));
+      const int reps = (launched++ < kChunkGrowAfter) ? 1 : kChunkReps;
+      for (int r = 0; r < reps; ++r)
+        RKX_HIP(hipGraphLaunch(gh.ge, stream));
       RKX_HIP(hipMemcpyAsync(&hs_pinned[slot], st_dev, sizeof(state), hipMemcpyDeviceToHost,
                              stream));
       RKX_HIP(hipEventRecord(ev[slot], stream));

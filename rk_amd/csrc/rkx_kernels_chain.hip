@@ -49,21 +49,23 @@ __global__ __launch_bounds__(kCB) void k_chain(const chain_inst *__restrict__ in
   const int tile = x8 * q + min(x8, rm) + (b >> 3);
   const int2 td = tiles ? tiles[tile] : make_int2(0, tile);
   const chain_inst &I = insts[td.x];
-  state *st = I.st;
-  if (st->done)
-    return;
-  const int cur = st->cur;
-  const double2 *__restrict__ y = reinterpret_cast<const double2 *>(cur ? I.y1 : I.y0);
-  double2 *__restrict__ yn = reinterpret_cast<double2 *>(cur ? I.y0 : I.y1);
+  double *const y0p = I.y0, *const y1p = I.y1;
   const int M = I.cells;
+  state *st = I.st;
+  const step_view sv = read_step(st);
+  if (sv.done)
+    return;
+  const int cur = sv.cur;
+  const double2 *__restrict__ y = reinterpret_cast<const double2 *>(cur ? y1p : y0p);
+  double2 *__restrict__ yn = reinterpret_cast<double2 *>(cur ? y0p : y1p);
   dev_ode p{};
   p.c0 = I.c0;
   p.d0 = I.d0;
   p.d1 = I.d1;
   p.c1 = I.c1;
   const double D = I.diff;
-  const dev_method *__restrict__ mt = st->meth;
-  const double h = st->h;                        // this step's h (uniform)
+  gmethod *__restrict__ mt = as_global(sv.meth);
+  const double h = sv.h;                         // this step's h (uniform)
 
   const int t = threadIdx.x;
 

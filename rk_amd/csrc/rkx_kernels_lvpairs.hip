@@ -30,9 +30,10 @@ __global__ __launch_bounds__(kPBlock) void k_lv_step(dev_ode p, const dev_coef *
 {
   __shared__ double red[kPBlock / 64];
   __shared__ unsigned int zred[kPBlock / 64];
-  if (st->done)
+  const step_view sv = read_step(st);
+  if (sv.done)
     return;                       // uniform: every block of the grid returns
-  const int cur = st->cur;
+  const int cur = sv.cur;
   const double *y = cur ? y1 : y0;
   double *ynew = cur ? y0 : y1;
   double qmax = 0.0;

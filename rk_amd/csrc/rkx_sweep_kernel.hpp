@@ -89,20 +89,23 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   // tile -> (instance, tile within the instance); no table: one instance
   const int2 td = tiles ? tiles[tile] : make_int2(0, tile);
   const sweep_inst &I = insts[td.x];
+  double *const y0p = I.y0, *const y1p = I.y1;
+  const int N = I.N;
+  const int R = I.R, OC = I.OC;
+  const int nwide = I.nwide, nbands = I.nbands;
   state *st = I.st;
-  if (st->done)
+  const step_view sv = read_step(st);
+  if (sv.done)
     return;                                      // uniform over the instance's tiles
   // uniform, said explicitly: y, y_new and every row address below stay in
   // scalar registers
-  const int cur = __builtin_amdgcn_readfirstlane(st->cur);
-  gconst *__restrict__ y = as_global(cur ? I.y1 : I.y0);
-  gmut *__restrict__ yn = as_global(cur ? I.y0 : I.y1);
-  const int N = I.N;
+  const int cur = __builtin_amdgcn_readfirstlane(sv.cur);
+  gconst *__restrict__ y = as_global(cur ? y1p : y0p);
+  gmut *__restrict__ yn = as_global(cur ? y0p : y1p);
   const int N2 = N + N;
   const double coef = I.coef;
-  const int R = I.R, OC = I.OC;
-  const dev_method *__restrict__ mt = st->meth;
-  const double h = st->h;                        // this step's h (uniform)
+  gmethod *__restrict__ mt = as_global(sv.meth);
+  const double h = sv.h;                         // this step's h (uniform)
   const int var = (threadIdx.x >= L) ? 1 : 0;   // wave-uniform: 0 = u, 1 = v
   const int cl = threadIdx.x - var * L;         // column lane within the strip
 
@@ -112,13 +115,13 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   // (whole waves, as L/2 is a multiple of 64); without a band 2p+1 those
   // waves repeat band 2p and store nothing.  Both halves run the full band
   // height, so every wave passes the same barriers.
-  const int nwt = I.nwide * I.nbands;
+  const int nwt = nwide * nbands;
   const bool paired = td.y >= nwt;
   int band, lo, lsub, c0, c1;
   bool live = true;
   if (!paired) {
-    const int strip = td.y / I.nbands;
-    band = td.y - strip * I.nbands;
+    const int strip = td.y / nbands;
+    band = td.y - strip * nbands;
     lo = 0;
     lsub = L;
     c0 = strip * OC;
@@ -126,13 +129,13 @@ __global__ __launch_bounds__(128 * K) __attribute__((amdgpu_waves_per_eu(P::occ)
   } else {
     const int half = __builtin_amdgcn_readfirstlane(cl >= L / 2 ? 1 : 0);
     band = 2 * (td.y - nwt) + half;
-    if (band >= I.nbands) {
-      band = I.nbands - 1;
+    if (band >= nbands) {
+      band = nbands - 1;
       live = false;
     }
     lo = half * (L / 2);
     lsub = L / 2;
-    c0 = I.nwide * OC;
+    c0 = nwide * OC;
     c1 = N;
   }
   const int cs = cl - lo;                       // column lane within the tile
